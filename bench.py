@@ -1,0 +1,241 @@
+"""Richardson-Lucy throughput on one MI355X.
+
+Workload: plain Richardson-Lucy (Clarity_MaximumLikelihoodDeconvolve
+semantics), 100 iterations on a float32 512x512x128 volume with a same-size
+kernel (FFT work domain 1024x1024x256): 16000 beads + 2400 random-walk
+filaments on background 10, anisotropic Gaussian kernel (sigma_xy 2.0,
+sigma_z 4.5, 10 % axial side lobes), Poisson noise -- synthetic, seeded.
+
+One step = one full deconvolution with the observation and kernel already
+resident in HBM: pad + kernel transform + energy, 100 RL iterations, clip
+into a device output buffer.  Metric: image voxels x iterations / s / 1e6.
+
+Also reported:
+  roofline      for the dominant kernel, from HIP events recorded on the
+                library stream around each launch inside the timed region;
+                algorithmic bytes per launch from the plan's own model.
+  cpu_baseline  the CPU oracle (the reference's CPU algorithm restated in C,
+                FFTW3-API FFTs via MKL when present, plan per FFT, float32
+                serial energy sum), timed on this host on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """Minimal HIP runtime calls for device buffers (plumbing only)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_int]
+
+    def malloc(self, n):
+        p = ctypes.c_void_p()
+        rc = self.L.hipMalloc(ctypes.byref(p), n)
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc({n}) failed: {rc}")
+        return p.value
+
+    def h2d(self, dst, arr):
+        rc = self.L.hipMemcpy(ctypes.c_void_p(dst), arr.ctypes.data_as(ctypes.c_void_p),
+                              arr.nbytes, 1)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy H2D failed: {rc}")
+
+    def d2h(self, arr, src):
+        rc = self.L.hipMemcpy(arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(src),
+                              arr.nbytes, 2)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2H failed: {rc}")
+
+    def free(self, p):
+        self.L.hipFree(ctypes.c_void_p(p))
+
+
+def make_inputs(api, shape, seed=3):
+    """Config-3 statistics on `shape`; observation convolved on the GPU."""
+    from clarity_amd import synth
+    rng = np.random.default_rng(seed)
+    base = (64, 256, 256)
+    f = np.prod(shape) / np.prod(base)
+    obj = synth.beads_and_filaments(shape, int(round(2000 * f)), int(round(300 * f)), rng)
+    psf = synth.gaussian_kernel(shape, 2.0, 4.5, side_lobe=0.1)
+    blurred = api.Convolve(obj, psf)
+    obs = synth.poisson(blurred, rng)
+    return obs, psf
+
+
+def cpu_baseline(obs, psf, iterations, threads):
+    """The oracle's RL on this host: `iterations` iterations of the full
+    volume, per-iteration loop time.  Reference cost structure (FFTW3 API
+    with a plan per call, buffer per convolution, float32 serial sum)."""
+    import oracle
+    backend = "own-fp64"
+    if oracle.set_fft_backend(oracle.FFT_FFTW_API):
+        backend = oracle.fft_backend_name()
+    oracle.set_reduce_mode(oracle.REDUCE_SERIAL_F32)
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    oracle.ml(obs, psf, iterations)
+    total = time.perf_counter() - t0
+    loop = oracle.last_loop_seconds()
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+    return loop, total, backend
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iterations", type=int, default=100)
+    ap.add_argument("--shape", type=str, default="128,512,512", help="image z,y,x")
+    ap.add_argument("--cpu-iterations", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this "
+              f"build (each box has one MI355X; the multi-GPU sharding is design-only).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from clarity_amd import api
+    shape = tuple(int(v) for v in args.shape.split(","))
+    api.Register()
+    log(f"[bench] device {api.DeviceName()}; image {shape}; building inputs")
+    obs, psf = make_inputs(api, shape)
+    hip = Hip()
+    d_obs = hip.malloc(obs.nbytes)
+    d_psf = hip.malloc(psf.nbytes)
+    d_out = hip.malloc(obs.nbytes)
+    hip.h2d(d_obs, obs)
+    hip.h2d(d_psf, psf)
+
+    plan = api.RLPlan(shape, psf.shape)
+    log(f"[bench] engine {plan.engine}")
+
+    def step():
+        plan.set_inputs_device(d_obs, d_psf)
+        plan.iterate(args.iterations)
+        plan.estimate_device(d_out)
+
+    for i in range(args.warmup):
+        step()
+        api.Synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+
+    kernels = plan.describe()
+    if kernels:
+        plan.set_timing(True)
+        plan.timings(reset=True)
+    api.Synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    api.Synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"[bench] {args.steps} steps in {elapsed:.3f} s")
+
+    vox = int(np.prod(shape))
+    value = vox * args.iterations * args.steps / elapsed / 1e6
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # roofline of the dominant kernel
+    roof = None
+    if kernels:
+        tm = plan.timings()
+        rows = []
+        for (name, byts, per_it), (cnt, ms) in zip(kernels, tm):
+            if cnt:
+                rows.append((ms, name, byts, cnt))
+        rows.sort(reverse=True)
+        ms, name, byts, cnt = rows[0]
+        avg_s = ms / cnt / 1e3
+        achieved = byts / avg_s / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tf):
+            traffic = json.load(open(tf)).get(name)
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes": byts,
+                "avg_launch_us": round(avg_s * 1e6, 2)}
+        for ms_, name_, byts_, cnt_ in rows:
+            log(f"[bench] {name_:24s} launches {cnt_:6d} avg {ms_ / cnt_ * 1e3:9.2f} us "
+                f"{byts_ / (ms_ / cnt_ / 1e3) / 1e9:8.1f} GB/s")
+        plan.set_timing(False)
+
+    # whole-iteration roofline against the dense model 4R + 18C
+    nz, ny, nx = (2 * s for s in shape)
+    R = 4.0 * nx * ny * nz
+    C = 8.0 * (nx // 2 + 1) * ny * nz
+    it_s = elapsed / (args.steps * args.iterations)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline: {args.cpu_iterations} iteration(s) on {threads} threads")
+        loop, total, backend = cpu_baseline(obs, psf, args.cpu_iterations, threads)
+        cpu = {"value": round(vox * args.cpu_iterations / loop / 1e6, 3),
+               "unit": "Mvoxel-iters/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_iterations} RL iteration(s) of the same "
+                         f"{shape[2]}x{shape[1]}x{shape[0]} volume (work "
+                         f"{nx}x{ny}x{nz}), loop time {loop:.2f} s of a "
+                         f"{total:.2f} s call; FFT {backend}, serial float32 sum"}
+        log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s")
+
+    line = {
+        "metric": "Mvoxel-iters/s RL 512x512x128 fp32",
+        "value": round(value, 2),
+        "unit": "Mvoxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"richardson-lucy {args.iterations} iterations, image "
+                               f"{shape[2]}x{shape[1]}x{shape[0]}, same-size kernel, "
+                               f"work {nx}x{ny}x{nz}",
+                   "engine": plan.engine, "iterations": args.iterations,
+                   "ms_per_iteration": round(1e3 * it_s, 4),
+                   "dense_model_GBps": round((4 * R + 18 * C) / it_s / 1e9, 1)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    for p in (d_obs, d_psf, d_out):
+        hip.free(p)
+    api.UnRegister()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,435 @@
+// Host drivers of the public API (except Richardson-Lucy, in rl_plan.hip):
+// convolution, Wiener, Jansen-van Cittert, blind ML (as written and
+// alternating), the FFT entry points, pad/clip and the I-divergence.  All
+// buffers stay on the device from upload to download; sums are float64 on
+// the device and never come back to the host inside a call.
+#include "common.hpp"
+#include "work.hpp"
+
+#include <cstring>
+
+namespace clarity {
+
+ClarityResult_t work_geometry(Work &w, const float *in, Clarity_Dim3 id,
+                              const float *kernel, Clarity_Dim3 kd,
+                              const float *out) {
+  if (!in || !kernel || !out) return CLARITY_INVALID_ARGUMENT;
+  if (!dims_ok(id) || !dims_ok(kd)) return CLARITY_INVALID_ARGUMENT;
+  w.id = dims_of(id);
+  w.kd = dims_of(kd);
+  long long wx = (long long)id.x + kd.x, wy = (long long)id.y + kd.y,
+            wz = (long long)id.z + kd.z;
+  if (wx > kMaxGenericN || wy > kMaxGenericN || wz > kMaxGenericN)
+    return CLARITY_FFT_FAILED;
+  w.wd = Dims{(int)wx, (int)wy, (int)wz};
+  w.IV = w.id.voxels();
+  w.KV = w.kd.voxels();
+  w.V = w.wd.voxels();
+  w.C = (long long)w.wd.z * w.wd.y * (w.wd.x / 2 + 1);
+  w.scale = inv_volume(w.wd.x, w.wd.y, w.wd.z);
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t work_pad_device(Work &w, const float *in_d,
+                                const float *kernel_d, hipStream_t s) {
+  CL_TRY(w.obs.alloc(sizeof(float) * w.V));
+  CL_TRY(w.psf.alloc(sizeof(float) * w.V));
+  const int zero[3] = {0, 0, 0};
+  const int kshift[3] = {-w.kd.x / 2, -w.kd.y / 2, -w.kd.z / 2};
+  CL_TRY(k_pad_shift(w.obs.f(), w.wd, in_d, w.id, zero, 0.f, s));
+  CL_TRY(k_pad_shift(w.psf.f(), w.wd, kernel_d, w.kd, kshift, 0.f, s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t work_upload(Work &w, const float *in, const float *kernel,
+                            hipStream_t s) {
+  DevBuf img, ker;
+  CL_TRY(img.alloc(sizeof(float) * w.IV));
+  CL_TRY(ker.alloc(sizeof(float) * w.KV));
+  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(work_pad_device(w, img.f(), ker.f(), s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t convolve_otf(const Work &w, const float *in, const float2 *otf,
+                             float *out, float2 *spec, hipStream_t s) {
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, in, spec, s));
+  CL_TRY(k_modulate(spec, otf, spec, w.C, w.scale, s));
+  return fft3d_c2r(w.wd.x, w.wd.y, w.wd.z, spec, out, s);
+}
+
+ClarityResult_t work_download(const Work &w, const float *src, float *host_out,
+                              hipStream_t s) {
+  DevBuf o;
+  CL_TRY(o.alloc(sizeof(float) * w.IV));
+  CL_TRY(k_clip(o.f(), w.id, src, w.wd, s));
+  CL_HIP(hipMemcpyAsync(host_out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+namespace {
+
+// reference Clarity_Convolve -> Clarity_ConvolveInternal
+// (src/Convolve.cxx:41-115,146-184)
+ClarityResult_t convolve(float *in, Clarity_Dim3 id, float *kernel,
+                         Clarity_Dim3 kd, float *out) {
+  Work w;
+  CL_TRY(work_geometry(w, in, id, kernel, kd, out));
+  hipStream_t s = rt().stream;
+  CL_TRY(work_upload(w, in, kernel, s));
+  DevBuf a, b, res;
+  CL_TRY(a.alloc(sizeof(float2) * w.C));
+  CL_TRY(b.alloc(sizeof(float2) * w.C));
+  CL_TRY(res.alloc(sizeof(float) * w.V));
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.obs.f(), a.c(), s));
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.psf.f(), b.c(), s));
+  CL_TRY(k_modulate(a.c(), b.c(), a.c(), w.C, w.scale, s));
+  CL_TRY(fft3d_c2r(w.wd.x, w.wd.y, w.wd.z, a.c(), res.f(), s));
+  return work_download(w, res.f(), out, s);
+}
+
+// reference src/WienerDeconvolve.cxx:69-122,211-290
+ClarityResult_t wiener(float *in, Clarity_Dim3 id, float *kernel,
+                       Clarity_Dim3 kd, float *out, float eps) {
+  Work w;
+  CL_TRY(work_geometry(w, in, id, kernel, kd, out));
+  hipStream_t s = rt().stream;
+  CL_TRY(work_upload(w, in, kernel, s));
+  DevBuf a, b, res;
+  CL_TRY(a.alloc(sizeof(float2) * w.C));
+  CL_TRY(b.alloc(sizeof(float2) * w.C));
+  CL_TRY(res.alloc(sizeof(float) * w.V));
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.obs.f(), a.c(), s));
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.psf.f(), b.c(), s));
+  CL_TRY(k_wiener(a.c(), b.c(), a.c(), w.C, w.scale, eps, s));
+  CL_TRY(fft3d_c2r(w.wd.x, w.wd.y, w.wd.z, a.c(), res.f(), s));
+  return work_download(w, res.f(), out, s);
+}
+
+// reference src/JansenVanCittertDeconvolve.cxx:122-192,311-389
+ClarityResult_t jvc(float *in, Clarity_Dim3 id, float *kernel, Clarity_Dim3 kd,
+                    float *out, int iterations) {
+  if (iterations < 0) return CLARITY_INVALID_ARGUMENT;
+  Work w;
+  CL_TRY(work_geometry(w, in, id, kernel, kd, out));
+  hipStream_t s = rt().stream;
+  // max over the unpadded input (:324), on the device
+  DevBuf img, ker, mparts, mval;
+  CL_TRY(img.alloc(sizeof(float) * w.IV));
+  CL_TRY(ker.alloc(sizeof(float) * w.KV));
+  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(mparts.alloc(sizeof(float) * kParts));
+  CL_TRY(mval.alloc(sizeof(float)));
+  CL_TRY(k_max_partials(img.f(), w.IV, mparts.f(), kParts, s));
+  CL_TRY(k_finish_max(mparts.f(), kParts, mval.f(), s));
+  float mx = 0.f;
+  CL_HIP(hipMemcpyAsync(&mx, mval.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  CL_TRY(work_pad_device(w, img.f(), ker.f(), s));
+  CL_HIP(hipStreamSynchronize(s));
+  img.release();
+  ker.release();
+  const float A = 0.5f * mx;
+  const float invASq = 1.0f / (A * A);
+  DevBuf otf, spec, ik, ok;
+  CL_TRY(otf.alloc(sizeof(float2) * w.C));
+  CL_TRY(spec.alloc(sizeof(float2) * w.C));
+  CL_TRY(ik.alloc(sizeof(float) * w.V));
+  CL_TRY(ok.alloc(sizeof(float) * w.V));
+  CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.psf.f(), otf.c(), s));
+  CL_TRY(convolve_otf(w, w.obs.f(), otf.c(), ok.f(), spec.c(), s));
+  CL_TRY(k_jvc(ik.f(), w.obs.f(), w.obs.f(), ok.f(), w.V, A, invASq, s));
+  for (int k = 1; k < iterations; k++) {
+    CL_TRY(convolve_otf(w, ik.f(), otf.c(), ok.f(), spec.c(), s));
+    CL_TRY(k_jvc(ik.f(), w.obs.f(), ik.f(), ok.f(), w.V, A, invASq, s));
+  }
+  return work_download(w, ik.f(), out, s);
+}
+
+// reference src/BlindMaximumLikelihoodDeconvolve.cxx:50-148,249-317 as
+// written: h renormalised with a float sum every iteration, image updated
+// without energy rescale, h otherwise unchanged.
+ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
+                         Clarity_Dim3 kd, float *out, int iterations) {
+  if (iterations < 0) return CLARITY_INVALID_ARGUMENT;
+  Work w;
+  CL_TRY(work_geometry(w, in, id, kernel, kd, out));
+  hipStream_t s = rt().stream;
+  CL_TRY(work_upload(w, in, kernel, s));
+  DevBuf hft, spec, o, tmp, parts;
+  CL_TRY(hft.alloc(sizeof(float2) * w.C));
+  CL_TRY(spec.alloc(sizeof(float2) * w.C));
+  CL_TRY(o.alloc(sizeof(float) * w.V));
+  CL_TRY(tmp.alloc(sizeof(float) * w.V));
+  CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
+  CL_HIP(hipMemcpyAsync(o.p, w.obs.p, sizeof(float) * w.V, hipMemcpyDeviceToDevice, s));
+  float *scal = reinterpret_cast<float *>(parts.d() + kParts);
+  for (int k = 0; k < iterations; k++) {
+    CL_TRY(k_normalize_f32sum(w.psf.f(), w.V, parts.d(), kParts, scal, s));
+    CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, w.psf.f(), hft.c(), s));
+    CL_TRY(convolve_otf(w, o.f(), hft.c(), tmp.f(), spec.c(), s));
+    CL_TRY(k_divide(tmp.f(), w.obs.f(), tmp.f(), 0.f, w.V, s));
+    CL_TRY(convolve_otf(w, tmp.f(), hft.c(), tmp.f(), spec.c(), s));
+    CL_TRY(k_multiply_sum(o.f(), o.f(), tmp.f(), w.V, parts.d(), kParts, s));
+  }
+  return work_download(w, o.f(), out, s);
+}
+
+// Alternating blind RL (see ClarityEx.h).
+ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
+                         Clarity_Dim3 kd, float *out, float *kernel_out,
+                         int outer, int inner) {
+  if (outer < 0 || inner < 0) return CLARITY_INVALID_ARGUMENT;
+  Work w;
+  CL_TRY(work_geometry(w, in, id, kernel, kd, out));
+  hipStream_t s = rt().stream;
+  CL_TRY(work_upload(w, in, kernel, s));
+  DevBuf hft, oft, spec, o, s1, s2, parts, scal;
+  CL_TRY(hft.alloc(sizeof(float2) * w.C));
+  CL_TRY(oft.alloc(sizeof(float2) * w.C));
+  CL_TRY(spec.alloc(sizeof(float2) * w.C));
+  CL_TRY(o.alloc(sizeof(float) * w.V));
+  CL_TRY(s1.alloc(sizeof(float) * w.V));
+  CL_TRY(s2.alloc(sizeof(float) * w.V));
+  CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
+  CL_TRY(scal.alloc(sizeof(float) * 4));  // E, sum, scale
+  float *E = scal.f(), *S = scal.f() + 1, *c = scal.f() + 2;
+  float *h = w.psf.f();
+  CL_HIP(hipMemcpyAsync(o.p, w.obs.p, sizeof(float) * w.V, hipMemcpyDeviceToDevice, s));
+  CL_TRY(k_normalize_fp64(h, w.V, parts.d(), kParts, s));
+  CL_TRY(k_sum_partials(w.obs.f(), w.V, parts.d(), kParts, s));
+  CL_TRY(k_finish_sum(parts.d(), kParts, E, nullptr, nullptr, s));
+  for (int it = 0; it < outer; it++) {
+    CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, h, hft.c(), s));
+    for (int k = 0; k < inner; k++) {
+      CL_TRY(convolve_otf(w, o.f(), hft.c(), s1.f(), spec.c(), s));
+      CL_TRY(k_divide(s1.f(), w.obs.f(), s1.f(), 0.f, w.V, s));
+      CL_TRY(convolve_otf(w, s1.f(), hft.c(), s2.f(), spec.c(), s));
+      CL_TRY(k_multiply_sum(o.f(), o.f(), s2.f(), w.V, parts.d(), kParts, s));
+      CL_TRY(k_finish_sum(parts.d(), kParts, S, E, c, s));
+      CL_TRY(k_scale_dev(o.f(), o.f(), w.V, c, s));
+    }
+    CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, o.f(), oft.c(), s));
+    for (int k = 0; k < inner; k++) {
+      CL_TRY(convolve_otf(w, h, oft.c(), s1.f(), spec.c(), s));
+      CL_TRY(k_divide(s1.f(), w.obs.f(), s1.f(), 0.f, w.V, s));
+      CL_TRY(convolve_otf(w, s1.f(), oft.c(), s2.f(), spec.c(), s));
+      CL_TRY(k_multiply_sum(h, h, s2.f(), w.V, parts.d(), kParts, s));
+      CL_TRY(k_normalize_fp64(h, w.V, parts.d(), kParts, s));
+    }
+  }
+  if (kernel_out) {
+    // undo the -k/2 cyclic shift, then take the kernel-sized low corner
+    DevBuf un, ko;
+    CL_TRY(un.alloc(sizeof(float) * w.V));
+    CL_TRY(ko.alloc(sizeof(float) * w.KV));
+    const int back[3] = {w.kd.x / 2, w.kd.y / 2, w.kd.z / 2};
+    CL_TRY(k_pad_shift(un.f(), w.wd, h, w.wd, back, 0.f, s));
+    CL_TRY(k_clip(ko.f(), w.kd, un.f(), w.wd, s));
+    CL_HIP(hipMemcpyAsync(kernel_out, ko.p, sizeof(float) * w.KV,
+                          hipMemcpyDeviceToHost, s));
+  }
+  return work_download(w, o.f(), out, s);
+}
+
+ClarityResult_t fft_r2c_api(int nx, int ny, int nz, float *in, float *out) {
+  if (!in || !out || nx <= 0 || ny <= 0 || nz <= 0) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  const long long V = (long long)nx * ny * nz;
+  const long long C = (long long)nz * ny * (nx / 2 + 1);
+  DevBuf a, b;
+  CL_TRY(a.alloc(sizeof(float) * V));
+  CL_TRY(b.alloc(sizeof(float2) * C));
+  CL_HIP(hipMemcpyAsync(a.p, in, sizeof(float) * V, hipMemcpyHostToDevice, s));
+  CL_TRY(fft3d_r2c(nx, ny, nz, a.f(), b.c(), s));
+  CL_HIP(hipMemcpyAsync(out, b.p, sizeof(float2) * C, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t fft_c2r_api(int nx, int ny, int nz, float *in, float *out) {
+  if (!in || !out || nx <= 0 || ny <= 0 || nz <= 0) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  const long long V = (long long)nx * ny * nz;
+  const long long C = (long long)nz * ny * (nx / 2 + 1);
+  DevBuf a, b;
+  CL_TRY(a.alloc(sizeof(float2) * C));
+  CL_TRY(b.alloc(sizeof(float) * V));
+  CL_HIP(hipMemcpyAsync(a.p, in, sizeof(float2) * C, hipMemcpyHostToDevice, s));
+  CL_TRY(fft3d_c2r(nx, ny, nz, a.c(), b.f(), s));
+  CL_HIP(hipMemcpyAsync(out, b.p, sizeof(float) * V, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t pad_api(float *dst, Clarity_Dim3 dd, float *src,
+                        Clarity_Dim3 sd, int shift[3], float fill) {
+  // reference src/ImagePadSpatialShift.cxx:38-40: NULL -> INVALID_ARGUMENT
+  if (!dst || !src || !shift) return CLARITY_INVALID_ARGUMENT;
+  if (!dims_ok(dd) || !dims_ok(sd)) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  Dims D = dims_of(dd), S = dims_of(sd);
+  DevBuf a, b;
+  CL_TRY(a.alloc(sizeof(float) * S.voxels()));
+  CL_TRY(b.alloc(sizeof(float) * D.voxels()));
+  CL_HIP(hipMemcpyAsync(a.p, src, sizeof(float) * S.voxels(), hipMemcpyHostToDevice, s));
+  CL_TRY(k_pad_shift(b.f(), D, a.f(), S, shift, fill, s));
+  CL_HIP(hipMemcpyAsync(dst, b.p, sizeof(float) * D.voxels(), hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t clip_api(float *dst, Clarity_Dim3 dd, float *src,
+                         Clarity_Dim3 sd) {
+  if (!dst || !src) return CLARITY_INVALID_ARGUMENT;  // src/ImageClip.cxx:34-36
+  if (!dims_ok(dd) || !dims_ok(sd)) return CLARITY_INVALID_ARGUMENT;
+  if (dd.x > sd.x || dd.y > sd.y || dd.z > sd.z) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  Dims D = dims_of(dd), S = dims_of(sd);
+  DevBuf a, b;
+  CL_TRY(a.alloc(sizeof(float) * S.voxels()));
+  CL_TRY(b.alloc(sizeof(float) * D.voxels()));
+  CL_HIP(hipMemcpyAsync(a.p, src, sizeof(float) * S.voxels(), hipMemcpyHostToDevice, s));
+  CL_TRY(k_clip(b.f(), D, a.f(), S, s));
+  CL_HIP(hipMemcpyAsync(dst, b.p, sizeof(float) * D.voxels(), hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t idiv_api(const float *g, const float *ghat, int64_t n,
+                         double *result) {
+  if (!g || !ghat || !result || n <= 0) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  DevBuf a, b, parts;
+  CL_TRY(a.alloc(sizeof(float) * n));
+  CL_TRY(b.alloc(sizeof(float) * n));
+  CL_TRY(parts.alloc(sizeof(double) * kParts));
+  CL_HIP(hipMemcpyAsync(a.p, g, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(b.p, ghat, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_TRY(k_idiv_partials(a.f(), b.f(), n, parts.d(), kParts, s));
+  std::vector<double> h(kParts);
+  CL_HIP(hipMemcpyAsync(h.data(), parts.p, sizeof(double) * kParts,
+                        hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  double acc = 0.0;
+  for (double v : h) acc += v;
+  *result = acc;
+  return CLARITY_SUCCESS;
+}
+
+// Zero-filling stubs kept for ABI completeness (reference
+// src/SmoothedJansenVanCittertDeconvolve.cxx:28-39,
+// src/IDivergenceDeconvolve.cxx:59-72).
+ClarityResult_t zero_stub(float *out, Clarity_Dim3 id) {
+  if (!out || !dims_ok(id)) return CLARITY_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(float) * (size_t)dims_of(id).voxels());
+  return CLARITY_SUCCESS;
+}
+
+}  // namespace
+}  // namespace clarity
+
+using namespace clarity;
+
+#define API_LOCK std::lock_guard<std::recursive_mutex> lk_(rt().mu)
+
+extern "C" {
+
+ClarityResult_t Clarity_Convolve(float *inImage, Clarity_Dim3 imageDim,
+                                 float *kernel, Clarity_Dim3 kernelDim,
+                                 float *outImage) {
+  API_LOCK;
+  return convolve(inImage, imageDim, kernel, kernelDim, outImage);
+}
+
+ClarityResult_t Clarity_WienerDeconvolve(float *inImage, Clarity_Dim3 imageDim,
+                                         float *kernelImage,
+                                         Clarity_Dim3 kernelDim,
+                                         float *outImage, float epsilon) {
+  API_LOCK;
+  return wiener(inImage, imageDim, kernelImage, kernelDim, outImage, epsilon);
+}
+
+ClarityResult_t Clarity_JansenVanCittertDeconvolve(
+    float *inImage, Clarity_Dim3 imageDim, float *kernelImage,
+    Clarity_Dim3 kernelDim, float *outImage, int iterations) {
+  API_LOCK;
+  return jvc(inImage, imageDim, kernelImage, kernelDim, outImage, iterations);
+}
+
+ClarityResult_t Clarity_BlindMaximumLikelihoodDeconvolve(
+    float *inImage, Clarity_Dim3 imageDim, float *kernelImage,
+    Clarity_Dim3 kernelDim, float *outImage, int iterations) {
+  API_LOCK;
+  return blind_ml(inImage, imageDim, kernelImage, kernelDim, outImage,
+                  iterations);
+}
+
+ClarityResult_t ClarityEx_BlindRL(float *inImage, Clarity_Dim3 imageDim,
+                                  float *kernelImage, Clarity_Dim3 kernelDim,
+                                  float *outImage, float *kernelOut, int outer,
+                                  int inner) {
+  API_LOCK;
+  return blind_rl(inImage, imageDim, kernelImage, kernelDim, outImage,
+                  kernelOut, outer, inner);
+}
+
+ClarityResult_t Clarity_SmoothedJansenVanCittertDeconvolve(
+    float *inImage, Clarity_Dim3 imageDim, float *kernelImage,
+    Clarity_Dim3 kernelDim, float *outImage, unsigned iterations,
+    unsigned smoothInterval, float *smoothSigma[3]) {
+  (void)inImage; (void)kernelImage; (void)kernelDim; (void)iterations;
+  (void)smoothInterval; (void)smoothSigma;
+  return zero_stub(outImage, imageDim);
+}
+
+ClarityResult_t Clarity_IDivergenceDeconvolve(float *inImage,
+                                              Clarity_Dim3 imageDim,
+                                              float *kernelImage,
+                                              Clarity_Dim3 kernelDim,
+                                              float *outImage) {
+  (void)inImage; (void)kernelImage; (void)kernelDim;
+  return zero_stub(outImage, imageDim);
+}
+
+ClarityResult_t Clarity_FFT_R2C_float(int nx, int ny, int nz, float *in,
+                                      float *out) {
+  API_LOCK;
+  return fft_r2c_api(nx, ny, nz, in, out);
+}
+
+ClarityResult_t Clarity_FFT_C2R_float(int nx, int ny, int nz, float *in,
+                                      float *out) {
+  API_LOCK;
+  return fft_c2r_api(nx, ny, nz, in, out);
+}
+
+ClarityResult_t Clarity_ImagePadSpatialShift(float *dst, Clarity_Dim3 dstDim,
+                                             float *src, Clarity_Dim3 srcDim,
+                                             int shift[3], float fillValue) {
+  API_LOCK;
+  return pad_api(dst, dstDim, src, srcDim, shift, fillValue);
+}
+
+ClarityResult_t Clarity_ImageClip(float *dst, Clarity_Dim3 dstDim, float *src,
+                                  Clarity_Dim3 srcDim) {
+  API_LOCK;
+  return clip_api(dst, dstDim, src, srcDim);
+}
+
+ClarityResult_t ClarityEx_IDivergence(const float *g, const float *ghat,
+                                      int64_t n, double *result) {
+  API_LOCK;
+  return idiv_api(g, ghat, n, result);
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+// Engine interface behind ClarityEx_RLPlan and Clarity_MaximumLikelihood*.
+#pragma once
+
+#include <memory>
+
+#include "common.hpp"
+#include "work.hpp"
+
+namespace clarity {
+
+class RLEngine {
+ public:
+  virtual ~RLEngine() = default;
+  virtual ClarityResult_t allocate() = 0;
+  // obs_d: imageDim floats, psf_d: kernelDim floats (device).  Computes the
+  // OTF and E_obs; leaves the estimate undefined until reset().
+  virtual ClarityResult_t set_inputs(const float *obs_d, const float *psf_d,
+                                     hipStream_t s) = 0;
+  // init_d: imageDim floats on the device, or nullptr for the observation.
+  virtual ClarityResult_t reset(const float *init_d, hipStream_t s) = 0;
+  virtual ClarityResult_t iterate(int n, hipStream_t s) = 0;
+  // Writes the energy-scaled estimate (imageDim floats) to out_d.
+  virtual ClarityResult_t estimate(float *out_d, hipStream_t s) = 0;
+  // Device address of the last rescale factor.
+  virtual const float *scale_ptr() const = 0;
+  virtual const char *name() const = 0;
+  virtual int describe(const char **names, double *bytes, int *launches,
+                       int cap) {
+    (void)names; (void)bytes; (void)launches; (void)cap;
+    return 0;
+  }
+  virtual ClarityResult_t set_timing(bool on) {
+    (void)on;
+    return CLARITY_INVALID_OPERATION;
+  }
+  virtual int timings(int *count, double *ms, int cap, bool reset) {
+    (void)count; (void)ms; (void)cap; (void)reset;
+    return 0;
+  }
+
+ protected:
+  Work w_;
+};
+
+std::unique_ptr<RLEngine> make_generic_rl(const Work &geom);
+// nullptr when the shape is not one the fused pipeline handles.
+std::unique_ptr<RLEngine> make_fast_rl(const Work &geom);
+
+}  // namespace clarity

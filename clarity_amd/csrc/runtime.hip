@@ -1,0 +1,183 @@
+// Library runtime: registration (reference src/Clarity.cxx:41-150), device
+// selection, the library stream and the device-memory cache.
+#include "common.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace clarity {
+
+Runtime &rt() {
+  static Runtime r;
+  return r;
+}
+
+bool ready() { return rt().refcount > 0 && rt().device >= 0; }
+
+ClarityResult_t from_hip(hipError_t e) {
+  if (e == hipSuccess) return CLARITY_SUCCESS;
+  if (e == hipErrorOutOfMemory) return CLARITY_DEVICE_OUT_OF_MEMORY;
+  if (std::getenv("CLARITY_LOG"))
+    std::fprintf(stderr, "clarity: HIP error %d (%s)\n", (int)e,
+                 hipGetErrorString(e));
+  return CLARITY_INVALID_OPERATION;
+}
+
+ClarityResult_t launch_check() {
+  hipError_t e = hipGetLastError();
+  return from_hip(e);
+}
+
+ClarityResult_t dev_alloc(void **p, size_t bytes) {
+  Runtime &r = rt();
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  r.alloc_count++;
+  // Fault injection for tests: CLARITY_FAULT_DEVICE_OOM=k fails every k-th
+  // device allocation (k = 1: all of them) as if HBM were exhausted.
+  if (const char *f = std::getenv("CLARITY_FAULT_DEVICE_OOM")) {
+    long long k = std::atoll(f);
+    if (k > 0 && r.alloc_count % k == 0) return CLARITY_DEVICE_OUT_OF_MEMORY;
+  }
+  auto it = r.cache.find(bytes);
+  if (it != r.cache.end()) {
+    *p = it->second;
+    r.cache_bytes -= it->first;
+    r.cache.erase(it);
+    return CLARITY_SUCCESS;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    // give cached blocks back and retry once
+    dev_cache_clear();
+    e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return CLARITY_DEVICE_OUT_OF_MEMORY;
+    }
+  }
+  return CLARITY_SUCCESS;
+}
+
+void dev_release(void *p, size_t bytes) {
+  if (!p) return;
+  Runtime &r = rt();
+  if (bytes == 0) bytes = 256;
+  // Keep freed blocks for reuse, up to a bound well inside 288 GB of HBM.
+  const size_t kCacheLimit = (size_t)64 << 30;
+  if (r.cache_bytes + bytes <= kCacheLimit) {
+    r.cache.emplace(bytes, p);
+    r.cache_bytes += bytes;
+  } else {
+    (void)hipFree(p);
+  }
+}
+
+void dev_cache_clear() {
+  Runtime &r = rt();
+  if (r.stream) (void)hipStreamSynchronize(r.stream);
+  for (auto &kv : r.cache) (void)hipFree(kv.second);
+  r.cache.clear();
+  r.cache_bytes = 0;
+}
+
+}  // namespace clarity
+
+using namespace clarity;
+
+extern "C" {
+
+Clarity_Dim3 Clarity_Dim3FromArray(int dimArray[3]) {
+  Clarity_Dim3 d;
+  d.x = dimArray[0];
+  d.y = dimArray[1];
+  d.z = dimArray[2];
+  return d;
+}
+
+ClarityResult_t Clarity_Register() {
+  Runtime &r = rt();
+  std::lock_guard<std::recursive_mutex> lk(r.mu);
+  if (r.refcount > 0) {
+    r.refcount++;
+    return CLARITY_SUCCESS;
+  }
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    std::fprintf(stderr, "clarity: no HIP device available\n");
+    return CLARITY_INVALID_OPERATION;
+  }
+  int dev = 0;
+  if (const char *s = std::getenv("CLARITY_DEVICE")) dev = std::atoi(s);
+  if (dev < 0 || dev >= count) return CLARITY_INVALID_ARGUMENT;
+  CL_HIP(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  CL_HIP(hipGetDeviceProperties(&prop, dev));
+  CL_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
+  r.device = dev;
+  r.device_name = prop.name;
+  r.alloc_count = 0;
+  if (std::getenv("CLARITY_LOG"))
+    std::fprintf(stderr, "clarity: device %d '%s' (%s), %zu MB\n", dev,
+                 prop.name, prop.gcnArchName,
+                 (size_t)(prop.totalGlobalMem >> 20));
+  r.refcount = 1;
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t Clarity_UnRegister() {
+  Runtime &r = rt();
+  std::lock_guard<std::recursive_mutex> lk(r.mu);
+  if (r.refcount <= 0) return CLARITY_SUCCESS;
+  if (--r.refcount > 0) return CLARITY_SUCCESS;
+  if (r.stream) (void)hipStreamSynchronize(r.stream);
+  for (auto &kv : r.twiddles) (void)hipFree(kv.second);
+  r.twiddles.clear();
+  dev_cache_clear();
+  if (r.stream) (void)hipStreamDestroy(r.stream);
+  r.stream = nullptr;
+  r.device = -1;
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t Clarity_Init() { return Clarity_Register(); }
+ClarityResult_t Clarity_Shutdown() { return Clarity_UnRegister(); }
+
+ClarityResult_t Clarity_SetNumberOfThreads(unsigned n) {
+  rt().host_threads = n;
+  return CLARITY_SUCCESS;
+}
+
+char *Clarity_ResultString(ClarityResult_t result) {
+  static char s_fft[] = "CLARITY_FFT_FAILED";
+  static char s_oom[] = "CLARITY_OUT_OF_MEMORY";
+  static char s_doom[] = "CLARITY_DEVICE_OUT_OF_MEMORY";
+  static char s_iop[] = "CLARITY_INVALID_OPERATION";
+  static char s_iarg[] = "CLARITY_INVALID_ARGUMENT";
+  static char s_ok[] = "CLARITY_SUCCESS";
+  static char s_unk[] = "<unknown Clarity result>";
+  switch (result) {
+    case CLARITY_FFT_FAILED: return s_fft;
+    case CLARITY_OUT_OF_MEMORY: return s_oom;
+    case CLARITY_DEVICE_OUT_OF_MEMORY: return s_doom;
+    case CLARITY_INVALID_OPERATION: return s_iop;
+    case CLARITY_INVALID_ARGUMENT: return s_iarg;
+    case CLARITY_SUCCESS: return s_ok;
+    default: return s_unk;
+  }
+}
+
+ClarityResult_t ClarityEx_Synchronize() {
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  CL_HIP(hipStreamSynchronize(rt().stream));
+  return CLARITY_SUCCESS;
+}
+
+const char *ClarityEx_DeviceName() { return rt().device_name.c_str(); }
+
+}  // extern "C"

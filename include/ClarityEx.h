@@ -1,0 +1,112 @@
+/*
+ * ClarityEx.h -- extensions of the MI355X Clarity library.
+ *
+ * None of these has a counterpart in the reference's public header; each
+ * comment names the reference behaviour it builds on.  Host-pointer
+ * functions are synchronous like the Clarity_* API.  The plan API keeps
+ * every buffer resident in HBM between calls, for callers that deconvolve
+ * many times on one shape (benchmarks, pipelines, warm starts).
+ */
+#ifndef __CLARITY_EX_H_
+#define __CLARITY_EX_H_
+
+#include <stdint.h>
+
+#include "Clarity.h"
+
+/* Richardson-Lucy from a caller-given initial estimate (imageDim floats,
+ * NULL = the observation, which is the reference's start,
+ * src/MaximumLikelihoodDeconvolve.cxx:137).  Everything else as
+ * Clarity_MaximumLikelihoodDeconvolve.  Used by warm starts and to resume. */
+C_FUNC_DEF ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveFrom(
+    float *inImage, Clarity_Dim3 imageDim, float *kernelImage,
+    Clarity_Dim3 kernelDim, float *initialImage, float *outImage,
+    int iterations);
+
+/* Alternating blind Richardson-Lucy.  Completes the kernel step that the
+ * reference leaves commented out (src/BlindMaximumLikelihoodDeconvolve.cxx:
+ * 89-96), with the reference's conventions (no conjugate, guard -> 0).  Per
+ * outer iteration: `inner` image RL updates with energy rescale, then
+ * `inner` kernel updates h' = h * (O (*) guard(obs / (O (*) h))), each
+ * followed by renormalising h to sum 1 in float64.  kernelOut (kernelDim
+ * floats, may be NULL) receives the final kernel. */
+C_FUNC_DEF ClarityResult_t ClarityEx_BlindRL(float *inImage,
+                                             Clarity_Dim3 imageDim,
+                                             float *kernelImage,
+                                             Clarity_Dim3 kernelDim,
+                                             float *outImage,
+                                             float *kernelOut, int outer,
+                                             int inner);
+
+/* I-divergence sum_i g ln(g/ghat) + ghat - g over n host floats, float64 on
+ * the device, 0 ln 0 := 0 (reference functional src/IDivergenceDeconvolve.cxx:
+ * 33-47 with alpha = 0). */
+C_FUNC_DEF ClarityResult_t ClarityEx_IDivergence(const float *g,
+                                                 const float *ghat, int64_t n,
+                                                 double *result);
+
+/* ---- resident Richardson-Lucy plan ------------------------------------ */
+
+typedef struct ClarityEx_RLPlan_t *ClarityEx_RLPlan;
+
+/* Creates a plan for one (imageDim, kernelDim) pair: allocates the HBM
+ * buffers and FFT twiddles.  No data yet. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanCreate(ClarityEx_RLPlan *plan,
+                                                  Clarity_Dim3 imageDim,
+                                                  Clarity_Dim3 kernelDim);
+
+/* Loads the observation (imageDim floats) and kernel (kernelDim floats) and
+ * computes the OTF.  onDevice != 0: both pointers are device pointers. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanSetInputs(ClarityEx_RLPlan plan,
+                                                     const float *inImage,
+                                                     const float *kernelImage,
+                                                     int onDevice);
+
+/* Sets the estimate to `initial` (imageDim floats; NULL = observation). */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanReset(ClarityEx_RLPlan plan,
+                                                 const float *initial,
+                                                 int onDevice);
+
+/* Runs `iterations` RL updates on the resident estimate, asynchronously on
+ * the library stream (no host synchronisation inside). */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanIterate(ClarityEx_RLPlan plan,
+                                                   int iterations);
+
+/* Copies the current (energy-scaled) estimate out; waits for the stream. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanGetEstimate(ClarityEx_RLPlan plan,
+                                                       float *out,
+                                                       int onDevice);
+
+/* Last rescale factor E_obs / sum(g') (float), read back from HBM. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanGetScale(ClarityEx_RLPlan plan,
+                                                    float *scale);
+
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanDestroy(ClarityEx_RLPlan plan);
+
+/* Describes the plan's kernels: writes up to `cap` entries, returns count.
+ * names[i] is a static string, bytes[i] the algorithmic HBM bytes one launch
+ * of that kernel moves, launches[i] its launches per RL iteration. */
+C_FUNC_DEF int ClarityEx_RLPlanDescribe(ClarityEx_RLPlan plan,
+                                        const char **names, double *bytes,
+                                        int *launches, int cap);
+
+/* Per-kernel timing with HIP events recorded on the library stream around
+ * each launch (0 = off, the default).  ClarityEx_RLPlanTimings returns for
+ * each kernel of ClarityEx_RLPlanDescribe the launch count and summed
+ * milliseconds since the last call with reset != 0. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanSetTiming(ClarityEx_RLPlan plan,
+                                                     int enable);
+C_FUNC_DEF int ClarityEx_RLPlanTimings(ClarityEx_RLPlan plan, int *count,
+                                       double *ms, int cap, int reset);
+
+/* Name of the engine running the plan: "fast" (fused power-of-two
+ * pipeline) or "generic". */
+C_FUNC_DEF const char *ClarityEx_RLPlanEngine(ClarityEx_RLPlan plan);
+
+/* Blocks until all work queued on the library stream has finished. */
+C_FUNC_DEF ClarityResult_t ClarityEx_Synchronize();
+
+/* Name of the GPU in use ("" before Clarity_Register). */
+C_FUNC_DEF const char *ClarityEx_DeviceName();
+
+#endif /* __CLARITY_EX_H_ */

@@ -1,0 +1,261 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each group checks the oracle against something that does not share its code:
+numpy's float64 FFT, closed forms, brute-force index maps, the committed
+golden fixtures (direct circular convolution in float64, no FFT), special
+cases of the divide guard, and invariants of the iterations.  A mistake such
+as a dropped term, a wrong sign, index or operand order fails one of them.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel_l2(a, b):
+    t = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a = np.asarray(a, t)
+    b = np.asarray(b, t)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(autouse=True)
+def _defaults():
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+    yield
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("shape", [(4, 6, 8), (5, 7, 9), (16, 32, 64), (3, 11, 13),
+                                   (2, 1, 1), (1, 1, 7), (6, 5, 67), (1, 2, 134)])
+def test_fft_vs_numpy(shape):
+    a = np.random.default_rng(1).standard_normal(shape).astype(np.float32)
+    F = oracle.fft_r2c(a)
+    assert rel_l2(F, np.fft.rfftn(a.astype(np.float64))) < 1e-7
+    G = (np.random.default_rng(2).standard_normal(F.shape)
+         + 1j * np.random.default_rng(3).standard_normal(F.shape)).astype(np.complex64)
+    ref = np.fft.irfftn(G.astype(np.complex128), s=shape, axes=(0, 1, 2)) * a.size
+    assert rel_l2(oracle.fft_c2r(G, shape[2]), ref) < 1e-7
+
+
+def test_fft_closed_forms():
+    d = np.zeros((4, 8, 16), np.float32)
+    d[0, 0, 0] = 1
+    assert np.allclose(oracle.fft_r2c(d), 1.0)
+    F = oracle.fft_r2c(np.full((4, 8, 16), 3.0, np.float32))
+    assert F[0, 0, 0] == 3.0 * 512
+    F[0, 0, 0] = 0
+    assert np.abs(F).max() < 1e-4
+    # sign convention: exp(-2 pi i k x / n) -> a shifted delta gives a
+    # negative phase ramp
+    d = np.zeros((1, 1, 8), np.float32)
+    d[0, 0, 1] = 1
+    F = oracle.fft_r2c(d)[0, 0]
+    assert np.allclose(F, np.exp(-2j * np.pi * np.arange(5) / 8), atol=1e-7)
+
+
+def test_fft_fftw_api_backend_agrees():
+    if not oracle.set_fft_backend(oracle.FFT_FFTW_API):
+        pytest.skip("no FFTW3 API library on this host")
+    try:
+        a = np.random.default_rng(4).standard_normal((8, 12, 20)).astype(np.float32)
+        F = oracle.fft_r2c(a)
+        assert rel_l2(F, np.fft.rfftn(a.astype(np.float64))) < 1e-6
+        assert rel_l2(oracle.fft_c2r(F, 20) / a.size, a) < 1e-6
+    finally:
+        oracle.set_fft_backend(oracle.FFT_OWN)
+
+
+# -------------------------------------------------------------- primitives
+def test_divide_guard_threshold():
+    f = np.float32
+    b = np.array([0.0, 1e-5, -1e-5, np.nextafter(f(1e-5), f(1)),
+                  -np.nextafter(f(1e-5), f(1)), 2.0, 1e-6], np.float32)
+    a = np.full(b.shape, 3.0, np.float32)
+    r = oracle.divide(a, b, value=-7.0)
+    # the float nearest 1e-5 lies below the double 1e-5, so it is guarded
+    assert list(r[:3]) == [-7.0, -7.0, -7.0]
+    assert r[3] == f(3.0) / b[3] and r[4] == f(3.0) / b[4]
+    assert r[5] == 1.5 and r[6] == -7.0
+
+
+def test_reduce_modes():
+    x = np.random.default_rng(5).uniform(0, 1000, 200001).astype(np.float32)
+    with oracle.reduce_mode(oracle.REDUCE_SERIAL_F32):
+        s32 = oracle.reduce_sum(x)
+    assert s32 == np.cumsum(x, dtype=np.float32)[-1]   # sequential float32
+    s64 = oracle.reduce_sum(x)
+    assert s64 == np.float32(math.fsum(x.astype(np.float64)))
+    # integers: exact in float64
+    y = np.arange(1000, dtype=np.float32)
+    assert oracle.reduce_sum(y) == 499500.0
+
+
+def test_modulate_special_operands():
+    nx = 8
+    a = (np.random.default_rng(6).standard_normal((2, 4, 5))
+         + 1j * np.random.default_rng(7).standard_normal((2, 4, 5))).astype(np.complex64)
+    one = np.ones_like(a)
+    s = np.float32(1.0) / np.float32(2 * 4 * 8)
+    assert np.array_equal(oracle.modulate(a, one, nx), (a * s).astype(np.complex64))
+    i = np.full_like(a, 1j)
+    out = oracle.modulate(a, i, nx)
+    assert np.array_equal(out.real, (-a.imag * s).astype(np.float32))
+    assert np.array_equal(out.imag, (a.real * s).astype(np.float32))
+
+
+def test_wiener_kernel_special_cases():
+    nx = 6
+    a = (np.random.default_rng(8).standard_normal((2, 3, 4))
+         + 1j * np.random.default_rng(9).standard_normal((2, 3, 4))).astype(np.complex64)
+    H = np.full_like(a, 2.0)
+    eps = 0.5
+    out = oracle.wiener_kernel(a, H, nx, eps)
+    ref = a * 2.0 / (4.0 + eps) / (2 * 3 * 6)
+    assert rel_l2(out, ref) < 1e-6
+    H = np.full_like(a, 1j)     # conj(i) = -i
+    out = oracle.wiener_kernel(a, H, nx, 0.0)
+    assert rel_l2(out, a * (-1j) / 36) < 1e-6
+
+
+def test_jvc_update_closed_form():
+    n = 64
+    rng = np.random.default_rng(10)
+    A = np.float32(5.0)
+    inv = np.float32(1.0) / (A * A)
+    inp = rng.uniform(0, 10, n).astype(np.float32)
+    ik = rng.uniform(0, 10, n).astype(np.float32)
+    ok = np.full(n, A, np.float32)                   # gamma = 1
+    out = oracle.jvc_update(inp, A, inv, ik, ok)
+    assert np.allclose(out, np.maximum(ik + (inp - A), 0), rtol=0, atol=1e-6)
+    ok = np.zeros(n, np.float32)                     # gamma = 0
+    assert np.array_equal(oracle.jvc_update(inp, A, inv, ik, ok), np.maximum(ik, 0))
+
+
+# ---------------------------------------------------------------- pad/clip
+def brute_pad(src, dst_shape, shift, fill):
+    """Straight from the definition: dst[d] = src[(d - shift) mod dst] if
+    that lands inside src (shifts within one period)."""
+    dz, dy, dx = dst_shape
+    sz, sy, sx = src.shape
+    out = np.full(dst_shape, fill, np.float32)
+    for k in range(dz):
+        for j in range(dy):
+            for i in range(dx):
+                sk, sj, si = (k - shift[2]) % dz, (j - shift[1]) % dy, (i - shift[0]) % dx
+                if sk < sz and sj < sy and si < sx:
+                    out[k, j, i] = src[sk, sj, si]
+    return out
+
+
+@pytest.mark.parametrize("src,dst,shift", [((3, 4, 5), (6, 8, 9), (0, 0, 0)),
+                                           ((3, 4, 5), (6, 8, 9), (-2, -2, -1)),
+                                           ((5, 5, 5), (7, 6, 9), (3, -4, 2)),
+                                           ((2, 2, 2), (2, 3, 4), (1, 1, 1))])
+def test_pad_shift_brute(src, dst, shift):
+    s = np.random.default_rng(11).uniform(size=src).astype(np.float32)
+    assert np.array_equal(oracle.pad_shift(s, dst, shift, -1.0), brute_pad(s, dst, shift, -1.0))
+
+
+def test_pad_shift_beyond_one_period_kept_as_written():
+    # A shift larger than the destination: the reference adds one period and
+    # then takes C's truncating remainder, so indices can stay negative and
+    # get the fill.  dx = 5, shift 12: di - 12 + 5 = -7..-3, and C gives
+    # -7 % 5 = -2, -6 % 5 = -1, -5 % 5 = 0, -4 % 5 = -4, -3 % 5 = -3: only
+    # di = 2 reads the source (si = 0).
+    s = np.ones((2, 2, 2), np.float32)
+    out = oracle.pad_shift(s, (2, 2, 5), (12, 0, 0), 0.0)
+    assert np.array_equal(out[0, 0], np.array([0, 0, 1, 0, 0], np.float32))
+
+
+def test_clip_brute():
+    s = np.random.default_rng(12).uniform(size=(5, 6, 7)).astype(np.float32)
+    assert np.array_equal(oracle.clip(s, (2, 3, 4)), s[:2, :3, :4])
+
+
+# -------------------------------------------------------- golden fixtures
+@pytest.mark.parametrize("name", ["ref_small", "ref_odd"])
+def test_drivers_vs_golden(name):
+    d = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    assert rel_l2(oracle.convolve(d["image"], d["kernel"]), d["convolve"]) < 1e-6
+    assert rel_l2(oracle.ml(d["obs"], d["kernel"], 1), d["rl1"]) < 1e-6
+    assert rel_l2(oracle.ml(d["obs"], d["kernel"], 5), d["rl5"]) < 2e-6
+    assert rel_l2(oracle.jvc(d["obs"], d["kernel"], 3), d["jvc3"]) < 1e-6
+    assert rel_l2(oracle.wiener(d["obs"], d["kernel"], 1e-3), d["wiener"]) < 1e-6
+
+
+# --------------------------------------------------------- RL invariants
+def _case(shape=(6, 10, 12), kshape=(3, 5, 5), seed=13):
+    rng = np.random.default_rng(seed)
+    obs = rng.uniform(1, 100, shape).astype(np.float32)
+    z, y, x = np.meshgrid(*[np.arange(n) - n // 2 for n in kshape], indexing="ij")
+    k = np.exp(-(x * x + y * y + 2 * z * z) / 3.0).astype(np.float32)
+    return obs, (k / k.sum()).astype(np.float32)
+
+
+def test_ml_delta_kernel_returns_observation():
+    obs, _ = _case()
+    k = np.zeros((3, 3, 3), np.float32)
+    k[1, 1, 1] = 1
+    out = oracle.ml(obs, k, 4)
+    assert rel_l2(out, obs) < 1e-6
+
+
+def test_ml_energy_scale_support():
+    obs, k = _case()
+    out, trace = oracle.ml_ex(obs, k, 6, want_trace=True)
+    assert abs(out.astype(np.float64).sum() / obs.astype(np.float64).sum() - 1) < 1e-6
+    assert np.all(out >= 0) and np.all(np.isfinite(trace))
+    out4 = oracle.ml(4 * obs, k, 6)
+    assert rel_l2(out4, 4 * out) < 1e-6
+    # zero iterations: observation back; negative rejected
+    assert np.array_equal(oracle.ml(obs, k, 0), obs)
+    with pytest.raises(RuntimeError):
+        oracle.ml(obs, k, -1)
+
+
+def test_ml_serial_f32_vs_fp64_modes_differ_only_by_rescale():
+    obs, k = _case((16, 32, 32), (5, 7, 7))
+    a = oracle.ml(obs, k, 3)
+    with oracle.reduce_mode(oracle.REDUCE_SERIAL_F32):
+        b = oracle.ml(obs, k, 3)
+    assert rel_l2(a, b) < 1e-5
+    assert rel_l2(a / a.sum(), b / b.sum()) < 1e-6
+
+
+def test_blind_as_written_equals_unscaled_rl():
+    # with a kernel already of sum 1, the as-written blind update is RL
+    # without the energy rescale: compare against ml_ex's per-step scale
+    obs, k = _case()
+    blind = oracle.blind_ml(obs, k, 1)
+    rl1, tr = oracle.ml_ex(obs, k, 1, want_trace=True)
+    assert rel_l2(blind * tr[0], rl1) < 1e-5
+
+
+def test_blind_rl_kernel_invariants():
+    obs, k = _case()
+    o, h = oracle.blind_rl(obs, k, 2, 2)
+    assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-5
+    assert np.all(h >= 0)
+    assert abs(o.astype(np.float64).sum() / obs.astype(np.float64).sum() - 1) < 1e-5
+    # zero outer iterations: observation and the (normalised) kernel back
+    o0, h0 = oracle.blind_rl(obs, k, 0, 3)
+    assert np.array_equal(o0, obs)
+    assert rel_l2(h0, k / k.astype(np.float64).sum()) < 1e-6
+
+
+def test_idivergence_closed_forms():
+    g = np.random.default_rng(14).uniform(0.1, 3, 1000).astype(np.float32)
+    assert abs(oracle.idivergence(g, g)) < 1e-9
+    gh = (g * 2).astype(np.float32)
+    ref = float(np.sum(g.astype(np.float64) * np.log(0.5) + g.astype(np.float64)))
+    assert abs(oracle.idivergence(g, gh) - ref) < 1e-9 * abs(ref)
+    z = np.zeros(10, np.float32)
+    assert oracle.idivergence(z, np.ones(10, np.float32)) == 10.0
