@@ -97,6 +97,7 @@ _SIGS = [
     ("ClarityEx_RLPlanEngine", ctypes.c_char_p, [ctypes.c_void_p]),
     ("ClarityEx_Synchronize", _R, []),
     ("ClarityEx_DeviceName", ctypes.c_char_p, []),
+    ("ClarityEx_InjectDeviceOOM", _R, [ctypes.c_int64]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGS]

@@ -27,6 +27,7 @@ struct Runtime {
   std::string device_name;
   unsigned host_threads = 1;
   long long alloc_count = 0;       // device allocations since Register
+  long long fault_every = 0;       // fail every k-th allocation (tests)
   std::multimap<size_t, void *> cache;  // freed device blocks by size
   size_t cache_bytes = 0;
   std::map<int, float2 *> twiddles;     // w_N^j tables, j < N

@@ -1,11 +1,801 @@
 // Fused, pruned Richardson-Lucy pipeline for power-of-two work shapes.
+//
+// The RL estimate and the ratio obs/s1 are zero outside the image corner of
+// the work domain (the estimate starts as the zero-padded observation and
+// every update multiplies it, reference src/MaximumLikelihoodDeconvolve.cxx:
+// 69-70,137), and s1, s2 are only ever read inside the corner.  So each 3D
+// transform reads only the nonzero corner and each inverse writes only the
+// corner: per RL iteration nine kernels stream
+//
+//   x_fwd   u (image rows)            -> S      R2C along x, rows y<iy, z<iz
+//   y_fwd   S (y<iy)                  -> S      FFT along y, all ny outputs
+//   z_mod   S (z<iz), OTF             -> S      FFT z, * OTF/N, IFFT z, z<iz kept
+//   y_inv   S                         -> S      IFFT along y, y<iy kept
+//   x_mid   S, obs                    -> S      C2R x, r = guard(obs/(c*s1)), R2C x
+//   y_fwd, z_mod, y_inv               (second convolution)
+//   x_last  S, u                      -> u      C2R x, u' = (c*u)*s2, fp64 partials
+//   finish  partials                  -> c' = E_obs / sum(u')
+//
+// with the estimate kept as u and its energy scale c in HBM (g = c*u), so no
+// iteration waits on the host.  The spectrum S has layout (z, kx, y), y
+// fastest, z < iz only; the OTF has layout (kz, kx, ky).  Every transform is
+// a register-resident Stockham FFT: a thread holds E elements of its pencil
+// (element t + T*e in v[e]), butterflies of radix E run in registers and
+// the stages exchange through LDS.  The x passes use the half-length complex
+// FFT of (x[2m] + i x[2m+1]) with the usual pack/unpack.
+#include "complex_ops.hpp"
 #include "rl_plan.hpp"
 
+#include <cstdio>
+#include <vector>
+
 namespace clarity {
+namespace {
+
+// ---------------------------------------------------------------- butterflies
+
+// cos / sin of 2*pi*j/16
+__device__ constexpr float kC16[16] = {
+    1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
+    0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f,
+    -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508978f,
+    0.0f, 0.38268343236508978f, 0.70710678118654752f, 0.92387953251128674f};
+__device__ constexpr float kS16[16] = {
+    0.0f, 0.38268343236508978f, 0.70710678118654752f, 0.92387953251128674f,
+    1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508978f,
+    0.0f, -0.38268343236508978f, -0.70710678118654752f, -0.92387953251128674f,
+    -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508978f};
+
+// a * exp(SIGN * 2 pi i j / 16) for a compile-time j
+template <int SIGN, int J>
+__device__ __forceinline__ float2 rot16(float2 a) {
+  constexpr int j = J & 15;
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (j == 4) {
+    return cmul_si(a, SIGN);
+  } else if constexpr (j == 8) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (j == 12) {
+    return cmul_si(a, -SIGN);
+  } else {
+    const float c = kC16[j];
+    const float s = SIGN * kS16[j];
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+// 16-point DFT: n = 4*n1 + n2, k = k1 + 4*k2; four DFT4 over n1, twiddle
+// w16^(n2*k1), four DFT4 over n2.
+template <int SIGN>
+__device__ __forceinline__ void dft16(float2 (&u)[16]) {
+  float2 a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+    float2 t[4] = {u[n2], u[4 + n2], u[8 + n2], u[12 + n2]};
+    dft4(t, SIGN);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) a[n2][k1] = t[k1];
+  }
+  a[1][1] = rot16<SIGN, 1>(a[1][1]);
+  a[1][2] = rot16<SIGN, 2>(a[1][2]);
+  a[1][3] = rot16<SIGN, 3>(a[1][3]);
+  a[2][1] = rot16<SIGN, 2>(a[2][1]);
+  a[2][2] = rot16<SIGN, 4>(a[2][2]);
+  a[2][3] = rot16<SIGN, 6>(a[2][3]);
+  a[3][1] = rot16<SIGN, 3>(a[3][1]);
+  a[3][2] = rot16<SIGN, 6>(a[3][2]);
+  a[3][3] = rot16<SIGN, 9>(a[3][3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4(t, SIGN);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) u[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft(float2 (&u)[R]) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
+    dft2(u[0], u[1]);
+  } else if constexpr (R == 4) {
+    dft4(u, SIGN);
+  } else if constexpr (R == 8) {
+    dft8(u, SIGN);
+  } else if constexpr (R == 16) {
+    dft16<SIGN>(u);
+  } else {
+    static_assert(R == 16, "unsupported radix");
+  }
+}
+
+template <int SIGN>
+__device__ __forceinline__ float2 tw_load(const float2 *__restrict__ tw, int idx) {
+  float2 w = tw[idx];
+  if (SIGN > 0) w.y = -w.y;
+  return w;
+}
+
+// ------------------------------------------------------------- LDS layouts
+
+// Row q of an x-pass tile, XOR-swizzled inside 16-element blocks so that a
+// column access (same pos, 16 consecutive rows) hits distinct banks.
+struct RowLds {
+  float2 *base;
+  int sw;
+  __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos ^ sw]; }
+};
+// Pencil q of an interleaved tile: element pos at pos*TP + q.
+template <int TP>
+struct InterLds {
+  float2 *base;
+  __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos * TP]; }
+};
+struct PencilLds {
+  float2 *base;
+  __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos]; }
+};
+
+// ------------------------------------------------------------ FFT engine
+
+// Stockham stages from span P on.  Thread t of a pencil holds x[t + T*e] in
+// v[e] on entry; on exit v[e] holds X[t + T*e] (natural order).  All threads
+// of the workgroup must call this together (it synchronises).
+template <int N, int E, int P, int SIGN, class L>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
+                                           const float2 *__restrict__ tw) {
+  constexpr int T = N / E;
+  constexpr int R = (N / P >= E) ? E : N / P;
+  constexpr bool LAST = (P * R == N);
+  constexpr int NB = E / R;
+  static_assert(E % R == 0, "radix must divide the per-thread count");
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    const int i = t + T * b;
+    const int k = i & (P - 1);
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = v[b + r * NB];
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++)
+        u[r] = cmul(u[r], tw_load<SIGN>(tw, r * k * (N / (P * R))));
+    }
+    dft<R, SIGN>(u);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; r++) v[b + r * NB] = u[r];
+    } else {
+      const int base = (i - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) lds(base + r * P) = u[r];
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = lds(t + T * e);
+    __syncthreads();
+    fft_stages<N, E, P * R, SIGN, L>(v, t, lds, tw);
+  }
+}
+
+struct Geom {
+  int ix, iy, iz;  // image
+  int nx, ny, nz;  // work (powers of two)
+  int hx;          // nx/2 + 1
+  long long sz;    // S / OTF stride of z: hx*ny
+};
+
+// ------------------------------------------------------------- x passes
+
+template <int M>
+__device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y) {
+  return ((long long)z * g.hx + k) * g.ny + y;
+}
+
+// Writes X[0..M] of the TP rows held (as Z = FFT_M(x_even + i x_odd)) in
+// LDS, to S with consecutive rows on consecutive lanes.
+template <int M, int TP, int NT>
+__device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restrict__ S,
+                                             const float2 *__restrict__ twN,
+                                             const Geom &g, int z, int y0) {
+  for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
+    const int qq = idx % TP;
+    const int k = idx / TP;
+    const int sw = qq & 15;
+    const float2 zk = lds[qq * M + ((k & (M - 1)) ^ sw)];
+    const float2 zm = lds[qq * M + (((M - k) & (M - 1)) ^ sw)];
+    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);   // Zk + conj(Zm)
+    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);   // Zk - conj(Zm)
+    const float2 wd = cmul(twN[k], d);                         // w^k d
+    // X = (a - i w^k d) / 2
+    const float2 X = make_float2(0.5f * (a.x + wd.y), 0.5f * (a.y - wd.x));
+    if (y0 + qq < g.iy) S[s_index<M>(g, z, k, y0 + qq)] = X;
+  }
+}
+
+// Loads X[0..M] of TP rows from S into LDS (row layout); the real parts of
+// X[0] and X[M] share slot 0 (their imaginary parts are ignored, as a
+// multi-dimensional C2R ignores them).
+template <int M, int TP, int NT>
+__device__ __forceinline__ void load_spectrum(float2 *lds, const float2 *__restrict__ S,
+                                              const Geom &g, int z, int y0) {
+  for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
+    const int qq = idx % TP;
+    const int k = idx / TP;
+    float2 X = make_float2(0.f, 0.f);
+    if (y0 + qq < g.iy) X = S[s_index<M>(g, z, k, y0 + qq)];
+    const int sw = qq & 15;
+    if (k == 0)
+      lds[qq * M + sw].x = X.x;
+    else if (k == M)
+      lds[qq * M + sw].y = X.x;
+    else
+      lds[qq * M + (k ^ sw)] = X;
+  }
+}
+
+// Forms Zc[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]) for the
+// thread's k = t + T*e; its inverse FFT is x[2m] + i x[2m+1].
+template <int M, int E>
+__device__ __forceinline__ void pack_inverse(float2 (&v)[E], const RowLds &L, int t,
+                                             const float2 *__restrict__ twN) {
+  constexpr int T = M / E;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int k = t + T * e;
+    float2 xk, xm;
+    if (k == 0) {
+      const float2 s = L(0);
+      xk = make_float2(s.x, 0.f);
+      xm = make_float2(s.y, 0.f);
+    } else {
+      xk = L(k);
+      xm = L(M - k);
+    }
+    const float2 a = make_float2(xk.x + xm.x, xk.y - xm.y);
+    const float2 d = make_float2(xk.x - xm.x, xk.y + xm.y);
+    float2 w = twN[k];
+    w.y = -w.y;                                  // w^-k
+    const float2 wd = cmul(w, d);
+    v[e] = make_float2(a.x - wd.y, a.y + wd.x);  // a + i w^-k d
+  }
+}
+
+template <int M, int E, int TP>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
+            const float2 *__restrict__ twM, const float2 *__restrict__ twN, Geom g) {
+  constexpr int T = M / E;
+  constexpr int NT = TP * T;
+  __shared__ float2 lds[TP * M];
+  const int q = threadIdx.x / T, t = threadIdx.x % T;
+  const int ntile = (g.iy + TP - 1) / TP;
+  const int z = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const int y = y0 + q;
+  const bool valid = y < g.iy;
+  const float *row = u + ((long long)z * g.iy + y) * g.ix;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int n0 = 2 * (t + T * e);
+    const float a = (valid && n0 < g.ix) ? row[n0] : 0.f;
+    const float b = (valid && n0 + 1 < g.ix) ? row[n0 + 1] : 0.f;
+    v[e] = make_float2(a, b);
+  }
+  const RowLds L{lds + q * M, q & 15};
+  fft_stages<M, E, 1, -1>(v, t, L, twM);
+#pragma unroll
+  for (int e = 0; e < E; e++) L(t + T * e) = v[e];
+  __syncthreads();
+  unpack_store<M, TP, NT>(lds, S, twN, g, z, y0);
+}
+
+// C2R of a spectrum tile -> v[e] = (x[2m], x[2m+1]), m = t + T*e
+template <int M, int E, int TP>
+__device__ __forceinline__ void inverse_rows(float2 (&v)[E], float2 *lds, const RowLds &L,
+                                             int t, const float2 *__restrict__ S,
+                                             const float2 *__restrict__ twM,
+                                             const float2 *__restrict__ twN, const Geom &g,
+                                             int z, int y0) {
+  constexpr int NT = TP * (M / E);
+  load_spectrum<M, TP, NT>(lds, S, g, z, y0);
+  __syncthreads();
+  pack_inverse<M, E>(v, L, t, twN);
+  __syncthreads();
+  fft_stages<M, E, 1, +1>(v, t, L, twM);
+}
+
+template <int M, int E, int TP>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
+            const float *__restrict__ cptr, const float2 *__restrict__ twM,
+            const float2 *__restrict__ twN, Geom g) {
+  constexpr int T = M / E;
+  constexpr int NT = TP * T;
+  __shared__ float2 lds[TP * M];
+  const int q = threadIdx.x / T, t = threadIdx.x % T;
+  const int ntile = (g.iy + TP - 1) / TP;
+  const int z = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const int y = y0 + q;
+  const bool valid = y < g.iy;
+  const RowLds L{lds + q * M, q & 15};
+  float2 v[E];
+  inverse_rows<M, E, TP>(v, lds, L, t, S, twM, twN, g, z, y0);
+  // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
+  // |s1| < 1e-5 (compared in double) gives 0, as the reference's ML step
+  // (src/ComputePrimitives.cxx:143-148, src/MaximumLikelihoodDeconvolve.cxx:62)
+  const float c = cptr[0];
+  const float *orow = obs + ((long long)z * g.iy + y) * g.ix;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int n0 = 2 * (t + T * e);
+    float r0 = 0.f, r1 = 0.f;
+    if (valid && n0 < g.ix) {
+      const float s = c * v[e].x;
+      r0 = ((double)fabsf(s) < 1e-5) ? 0.f : orow[n0] / s;
+    }
+    if (valid && n0 + 1 < g.ix) {
+      const float s = c * v[e].y;
+      r1 = ((double)fabsf(s) < 1e-5) ? 0.f : orow[n0 + 1] / s;
+    }
+    v[e] = make_float2(r0, r1);
+  }
+  fft_stages<M, E, 1, -1>(v, t, L, twM);
+#pragma unroll
+  for (int e = 0; e < E; e++) L(t + T * e) = v[e];
+  __syncthreads();
+  unpack_store<M, TP, NT>(lds, S, twN, g, z, y0);
+}
+
+template <int M, int E, int TP>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
+             const float *__restrict__ cptr, double *__restrict__ partials,
+             const float2 *__restrict__ twM, const float2 *__restrict__ twN, Geom g) {
+  constexpr int T = M / E;
+  constexpr int NT = TP * T;
+  __shared__ float2 lds[TP * M];
+  const int q = threadIdx.x / T, t = threadIdx.x % T;
+  const int ntile = (g.iy + TP - 1) / TP;
+  const int z = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const int y = y0 + q;
+  const bool valid = y < g.iy;
+  const RowLds L{lds + q * M, q & 15};
+  float2 v[E];
+  inverse_rows<M, E, TP>(v, lds, L, t, S, twM, twN, g, z, y0);
+  // u' = (c * u) * s2: the reference's g' = g * s2 with g = c * u stored
+  // rounded (src/MaximumLikelihoodDeconvolve.cxx:69-78)
+  const float c = cptr[0];
+  float *urow = u + ((long long)z * g.iy + y) * g.ix;
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int n0 = 2 * (t + T * e);
+    if (valid && n0 < g.ix) {
+      const float gk = c * urow[n0];
+      const float un = gk * v[e].x;
+      urow[n0] = un;
+      acc += (double)un;
+    }
+    if (valid && n0 + 1 < g.ix) {
+      const float gk = c * urow[n0 + 1];
+      const float un = gk * v[e].y;
+      urow[n0 + 1] = un;
+      acc += (double)un;
+    }
+  }
+  double *sh = reinterpret_cast<double *>(lds);
+  __syncthreads();
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// ------------------------------------------------------------- y and z
+
+template <int N, int E, int PPW, int SIGN>
+__global__ void __launch_bounds__(PPW *(N / E))
+    k_y(float2 *__restrict__ S, const float2 *__restrict__ tw, long long npencils,
+        int Lin, int Lout) {
+  constexpr int T = N / E;
+  __shared__ float2 lds[PPW * N];
+  const int pl = threadIdx.x / T, t = threadIdx.x % T;
+  const long long p = (long long)blockIdx.x * PPW + pl;
+  const bool valid = p < npencils;
+  float2 *base = S + p * N;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int j = t + T * e;
+    v[e] = (valid && j < Lin) ? base[j] : make_float2(0.f, 0.f);
+  }
+  const PencilLds L{lds + pl * N};
+  fft_stages<N, E, 1, SIGN>(v, t, L, tw);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int j = t + T * e;
+    if (valid && j < Lout) base[j] = v[e];
+  }
+}
+
+template <int N, int E, int TP>
+__global__ void __launch_bounds__(TP *(N / E))
+    k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
+        const float2 *__restrict__ tw, Geom g, float scale) {
+  constexpr int T = N / E;
+  __shared__ float2 lds[TP * N];
+  const int q = threadIdx.x % TP, t = threadIdx.x / TP;
+  const int ntile = g.ny / TP;
+  const int kx = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const long long col = (long long)kx * g.ny + y0 + q;
+  float2 v[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int z = t + T * e;
+    v[e] = (z < g.iz) ? S[z * g.sz + col] : make_float2(0.f, 0.f);
+  }
+  const InterLds<TP> L{lds + q};
+  fft_stages<N, E, 1, -1>(v, t, L, tw);
+  // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int kz = t + T * e;
+    const float2 h = otf[kz * g.sz + col];
+    const float2 a = v[e];
+    v[e] = make_float2(scale * (a.x * h.x - a.y * h.y), scale * (a.x * h.y + a.y * h.x));
+  }
+  fft_stages<N, E, 1, +1>(v, t, L, tw);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int z = t + T * e;
+    if (z < g.iz) S[z * g.sz + col] = v[e];
+  }
+}
+
+// ------------------------------------------------------------ dispatch
+
+enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KNUM };
+const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod", "y_inv",
+                                  "x_mid", "x_last", "finish"};
+
+constexpr int kTPx = 16;  // rows per x tile: 16 consecutive y -> 128 B
+constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
+
+struct Launch {
+  unsigned grid = 0, block = 0;
+};
+
+template <int M>
+struct XCfg;
+template <> struct XCfg<64> { static constexpr int E = 8; };
+template <> struct XCfg<128> { static constexpr int E = 8; };
+template <> struct XCfg<256> { static constexpr int E = 16; };
+template <> struct XCfg<512> { static constexpr int E = 8; };
+
+template <int N>
+struct YCfg;
+template <> struct YCfg<128> { static constexpr int E = 8, PPW = 16; };
+template <> struct YCfg<256> { static constexpr int E = 16, PPW = 16; };
+template <> struct YCfg<512> { static constexpr int E = 8, PPW = 4; };
+template <> struct YCfg<1024> { static constexpr int E = 16, PPW = 4; };
+template <> struct YCfg<2048> { static constexpr int E = 16, PPW = 2; };
+
+template <int N>
+struct ZCfg;
+template <> struct ZCfg<64> { static constexpr int E = 8; };
+template <> struct ZCfg<128> { static constexpr int E = 8; };
+template <> struct ZCfg<256> { static constexpr int E = 16; };
+template <> struct ZCfg<512> { static constexpr int E = 16; };
+
+bool x_supported(int M) { return M == 64 || M == 128 || M == 256 || M == 512; }
+bool y_supported(int N) {
+  return N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048;
+}
+bool z_supported(int N) { return N == 64 || N == 128 || N == 256 || N == 512; }
+
+struct Bufs {
+  float *u, *obs, *scal;  // scal: E, S, c
+  float2 *S, *otf;
+  double *parts;
+  const float2 *twM, *twN, *twY, *twZ;
+};
+
+template <int M>
+void launch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  constexpr int E = XCfg<M>::E;
+  constexpr int NT = kTPx * (M / E);
+  const unsigned grid = (unsigned)g.iz * ((g.iy + kTPx - 1) / kTPx);
+  if (kind == KX_FWD)
+    hipLaunchKernelGGL((k_x_fwd<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+                       b.twM, b.twN, g);
+  else if (kind == KX_MID)
+    hipLaunchKernelGGL((k_x_mid<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+                       b.scal + 2, b.twM, b.twN, g);
+  else
+    hipLaunchKernelGGL((k_x_last<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+                       b.scal + 2, b.parts, b.twM, b.twN, g);
+}
+
+void dispatch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  switch (g.nx / 2) {
+    case 64: launch_x<64>(kind, g, b, s); break;
+    case 128: launch_x<128>(kind, g, b, s); break;
+    case 256: launch_x<256>(kind, g, b, s); break;
+    case 512: launch_x<512>(kind, g, b, s); break;
+  }
+}
+
+template <int N>
+void launch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
+  constexpr int E = YCfg<N>::E, PPW = YCfg<N>::PPW;
+  const long long np = (long long)g.iz * g.hx;
+  const unsigned grid = (unsigned)((np + PPW - 1) / PPW);
+  if (fwd)
+    hipLaunchKernelGGL((k_y<N, E, PPW, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+                       b.twY, np, g.iy, N);
+  else
+    hipLaunchKernelGGL((k_y<N, E, PPW, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+                       b.twY, np, N, g.iy);
+}
+
+void dispatch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
+  switch (g.ny) {
+    case 128: launch_y<128>(fwd, g, b, s); break;
+    case 256: launch_y<256>(fwd, g, b, s); break;
+    case 512: launch_y<512>(fwd, g, b, s); break;
+    case 1024: launch_y<1024>(fwd, g, b, s); break;
+    case 2048: launch_y<2048>(fwd, g, b, s); break;
+  }
+}
+
+template <int N>
+void launch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  constexpr int E = ZCfg<N>::E;
+  const unsigned grid = (unsigned)(g.hx * (g.ny / kTPz));
+  hipLaunchKernelGGL((k_z<N, E, kTPz>), dim3(grid), dim3(kTPz * (N / E)), 0, s, b.S,
+                     b.otf, b.twZ, g, scale);
+}
+
+void dispatch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  switch (g.nz) {
+    case 64: launch_z<64>(g, b, scale, s); break;
+    case 128: launch_z<128>(g, b, scale, s); break;
+    case 256: launch_z<256>(g, b, scale, s); break;
+    case 512: launch_z<512>(g, b, scale, s); break;
+  }
+}
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// --------------------------------------------------------------- engine
+
+class FastRL : public RLEngine {
+ public:
+  explicit FastRL(const Work &geom) {
+    w_.set_geometry(geom);
+    g_.ix = geom.id.x; g_.iy = geom.id.y; g_.iz = geom.id.z;
+    g_.nx = geom.wd.x; g_.ny = geom.wd.y; g_.nz = geom.wd.z;
+    g_.hx = g_.nx / 2 + 1;
+    g_.sz = (long long)g_.hx * g_.ny;
+    nparts_ = g_.iz * ((g_.iy + kTPx - 1) / kTPx);
+  }
+  ~FastRL() override {
+    for (auto &ev : ev_pool_) {
+      (void)hipEventDestroy(ev.first);
+      (void)hipEventDestroy(ev.second);
+    }
+  }
+
+  static bool supports(const Work &w) {
+    return pow2(w.wd.x) && pow2(w.wd.y) && pow2(w.wd.z) && x_supported(w.wd.x / 2) &&
+           y_supported(w.wd.y) && z_supported(w.wd.z) && w.wd.y >= kTPz;
+  }
+
+  ClarityResult_t allocate() override {
+    CL_TRY(obs_.alloc(sizeof(float) * w_.IV));
+    CL_TRY(u_.alloc(sizeof(float) * w_.IV));
+    CL_TRY(S_.alloc(sizeof(float2) * (size_t)g_.iz * g_.sz));
+    CL_TRY(otf_.alloc(sizeof(float2) * (size_t)g_.nz * g_.sz));
+    CL_TRY(parts_.alloc(sizeof(double) * (size_t)(nparts_ > kParts ? nparts_ : kParts)));
+    CL_TRY(scal_.alloc(sizeof(float) * 4));
+    AxisPlan pm, pn, py, pz;
+    CL_TRY(axis_plan(g_.nx / 2, &pm));
+    CL_TRY(axis_plan(g_.nx, &pn));
+    CL_TRY(axis_plan(g_.ny, &py));
+    CL_TRY(axis_plan(g_.nz, &pz));
+    b_.twM = pm.tw; b_.twN = pn.tw; b_.twY = py.tw; b_.twZ = pz.tw;
+    b_.u = u_.f(); b_.obs = obs_.f(); b_.scal = scal_.f();
+    b_.S = S_.c(); b_.otf = otf_.c(); b_.parts = parts_.d();
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t set_inputs(const float *obs_d, const float *psf_d,
+                             hipStream_t s) override {
+    CL_HIP(hipMemcpyAsync(obs_.p, obs_d, sizeof(float) * w_.IV,
+                          hipMemcpyDeviceToDevice, s));
+    // OTF = R2C of the padded, -k/2 shifted kernel, laid out (kz, kx, ky)
+    {
+      DevBuf pad;
+      CL_TRY(pad.alloc(sizeof(float) * w_.V));
+      const int kshift[3] = {-w_.kd.x / 2, -w_.kd.y / 2, -w_.kd.z / 2};
+      CL_TRY(k_pad_shift(pad.f(), w_.wd, psf_d, w_.kd, kshift, 0.f, s));
+      AxisPlan px, py, pz;
+      CL_TRY(axis_plan(g_.nx, &px));
+      CL_TRY(axis_plan(g_.ny, &py));
+      CL_TRY(axis_plan(g_.nz, &pz));
+      PencilGeom gx;  // rows (z, y) -> element kx at stride ny
+      gx.nA = g_.nz; gx.nB = g_.ny;
+      gx.ia = (long long)g_.ny * g_.nx; gx.ib = g_.nx; gx.ie = 1;
+      gx.oa = g_.sz; gx.ob = 1; gx.oe = g_.ny;
+      gx.Lin = g_.nx; gx.Lout = g_.hx;
+      CL_TRY(fft_r2c_rows(pad.f(), otf_.c(), px, gx, s));
+      PencilGeom gy;  // pencils (z, kx) contiguous along y
+      gy.nA = g_.nz; gy.nB = g_.hx;
+      gy.ia = gy.oa = g_.sz; gy.ib = gy.ob = g_.ny; gy.ie = gy.oe = 1;
+      gy.Lin = gy.Lout = g_.ny;
+      CL_TRY(fft_c2c_pass(otf_.c(), otf_.c(), py, gy, -1, s));
+      PencilGeom gz;  // pencils (kx, ky) along z
+      gz.nA = 1; gz.nB = g_.sz;
+      gz.ia = gz.oa = 0; gz.ib = gz.ob = 1; gz.ie = gz.oe = g_.sz;
+      gz.Lin = gz.Lout = g_.nz;
+      CL_TRY(fft_c2c_pass(otf_.c(), otf_.c(), pz, gz, -1, s));
+      CL_HIP(hipStreamSynchronize(s));  // pad is released on return
+    }
+    // E_obs: the padded observation's sum (zeros outside the corner)
+    CL_TRY(k_sum_partials(obs_.f(), w_.IV, parts_.d(), kParts, s));
+    CL_TRY(k_finish_sum(parts_.d(), kParts, scal_.f(), nullptr, nullptr, s));
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t reset(const float *init_d, hipStream_t s) override {
+    CL_HIP(hipMemcpyAsync(u_.p, init_d ? init_d : obs_.p, sizeof(float) * w_.IV,
+                          hipMemcpyDeviceToDevice, s));
+    CL_TRY(k_fill(scal_.f() + 2, 1, 1.0f, s));
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t iterate(int n, hipStream_t s) override {
+    for (int it = 0; it < n; it++) {
+      CL_TRY(run(KX_FWD, s));
+      for (int half = 0; half < 2; half++) {
+        CL_TRY(run(KY_FWD, s));
+        CL_TRY(run(KZ_MOD, s));
+        CL_TRY(run(KY_INV, s));
+        CL_TRY(run(half == 0 ? KX_MID : KX_LAST, s));
+      }
+      CL_TRY(run(KFINISH, s));
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t estimate(float *out_d, hipStream_t s) override {
+    return k_scale_dev(out_d, u_.f(), w_.IV, scal_.f() + 2, s);
+  }
+
+  const float *scale_ptr() const override { return scal_.f() + 2; }
+  const char *name() const override { return "fast"; }
+
+  int describe(const char **names, double *bytes, int *launches,
+               int cap) override {
+    const double f = 4.0, c = 8.0;
+    const double IV = (double)w_.IV;
+    const double Sp = (double)g_.iz * g_.hx * g_.iy;  // spectrum, y < iy
+    const double Sf = (double)g_.iz * g_.hx * g_.ny;  // spectrum, all ky
+    const double O = (double)g_.nz * g_.hx * g_.ny;   // OTF
+    const double b[KNUM] = {
+        f * IV + c * Sp,                // x_fwd
+        c * Sp + c * Sf,                // y_fwd
+        c * Sf + c * O + c * Sf,        // z_mod
+        c * Sf + c * Sp,                // y_inv
+        c * Sp + f * IV + c * Sp,       // x_mid
+        c * Sp + f * IV + f * IV,       // x_last
+        8.0 * nparts_ + 12.0,           // finish
+    };
+    const int per_it[KNUM] = {1, 2, 2, 2, 1, 1, 1};
+    int n = 0;
+    for (int k = 0; k < KNUM && n < cap; k++, n++) {
+      names[n] = kNames[k];
+      bytes[n] = b[k];
+      launches[n] = per_it[k];
+    }
+    return n;
+  }
+
+  ClarityResult_t set_timing(bool on) override {
+    timing_ = on;
+    return CLARITY_SUCCESS;
+  }
+
+  int timings(int *count, double *ms, int cap, bool reset) override {
+    Runtime &r = rt();
+    if (r.stream) (void)hipStreamSynchronize(r.stream);
+    for (auto &rec : pending_) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, ev_pool_[rec.second].first,
+                              ev_pool_[rec.second].second) == hipSuccess) {
+        cnt_[rec.first]++;
+        ms_[rec.first] += t;
+      }
+    }
+    pending_.clear();
+    ev_next_ = 0;
+    int n = 0;
+    for (int k = 0; k < KNUM && n < cap; k++, n++) {
+      count[n] = cnt_[k];
+      ms[n] = ms_[k];
+    }
+    if (reset) {
+      for (int k = 0; k < KNUM; k++) {
+        cnt_[k] = 0;
+        ms_[k] = 0.0;
+      }
+    }
+    return n;
+  }
+
+ private:
+  ClarityResult_t run(Kind kind, hipStream_t s) {
+    int slot = -1;
+    if (timing_) {
+      if (ev_next_ == ev_pool_.size()) {
+        hipEvent_t a, b;
+        CL_HIP(hipEventCreate(&a));
+        CL_HIP(hipEventCreate(&b));
+        ev_pool_.push_back({a, b});
+      }
+      slot = (int)ev_next_++;
+      CL_HIP(hipEventRecord(ev_pool_[slot].first, s));
+    }
+    const float scale = inv_volume(g_.nx, g_.ny, g_.nz);
+    switch (kind) {
+      case KX_FWD:
+      case KX_MID:
+      case KX_LAST: dispatch_x(kind, g_, b_, s); break;
+      case KY_FWD: dispatch_y(true, g_, b_, s); break;
+      case KY_INV: dispatch_y(false, g_, b_, s); break;
+      case KZ_MOD: dispatch_z(g_, b_, scale, s); break;
+      case KFINISH:
+        CL_TRY(k_finish_sum(b_.parts, nparts_, b_.scal + 1, b_.scal, b_.scal + 2, s));
+        break;
+      default: break;
+    }
+    CL_TRY(launch_check());
+    if (timing_) {
+      CL_HIP(hipEventRecord(ev_pool_[slot].second, s));
+      pending_.push_back({kind, slot});
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  Geom g_{};
+  Bufs b_{};
+  int nparts_ = 0;
+  DevBuf obs_, u_, S_, otf_, parts_, scal_;
+  bool timing_ = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+  size_t ev_next_ = 0;
+  std::vector<std::pair<int, int>> pending_;
+  int cnt_[KNUM] = {};
+  double ms_[KNUM] = {};
+};
+
+}  // namespace
 
 std::unique_ptr<RLEngine> make_fast_rl(const Work &geom) {
-  (void)geom;
-  return nullptr;
+  if (!FastRL::supports(geom)) return nullptr;
+  return std::unique_ptr<RLEngine>(new FastRL(geom));
 }
 
 }  // namespace clarity

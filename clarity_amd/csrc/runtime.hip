@@ -34,12 +34,11 @@ ClarityResult_t dev_alloc(void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
   r.alloc_count++;
-  // Fault injection for tests: CLARITY_FAULT_DEVICE_OOM=k fails every k-th
-  // device allocation (k = 1: all of them) as if HBM were exhausted.
-  if (const char *f = std::getenv("CLARITY_FAULT_DEVICE_OOM")) {
-    long long k = std::atoll(f);
-    if (k > 0 && r.alloc_count % k == 0) return CLARITY_DEVICE_OUT_OF_MEMORY;
-  }
+  // Fault injection (ClarityEx_InjectDeviceOOM, or CLARITY_FAULT_DEVICE_OOM
+  // read at Register): every k-th device allocation fails as if HBM were
+  // exhausted.
+  if (r.fault_every > 0 && r.alloc_count % r.fault_every == 0)
+    return CLARITY_DEVICE_OUT_OF_MEMORY;
   auto it = r.cache.find(bytes);
   if (it != r.cache.end()) {
     *p = it->second;
@@ -122,6 +121,9 @@ ClarityResult_t Clarity_Register() {
   r.device = dev;
   r.device_name = prop.name;
   r.alloc_count = 0;
+  r.fault_every = 0;
+  if (const char *f = std::getenv("CLARITY_FAULT_DEVICE_OOM"))
+    r.fault_every = std::atoll(f);
   if (std::getenv("CLARITY_LOG"))
     std::fprintf(stderr, "clarity: device %d '%s' (%s), %zu MB\n", dev,
                  prop.name, prop.gcnArchName,
@@ -179,5 +181,14 @@ ClarityResult_t ClarityEx_Synchronize() {
 }
 
 const char *ClarityEx_DeviceName() { return rt().device_name.c_str(); }
+
+ClarityResult_t ClarityEx_InjectDeviceOOM(int64_t every) {
+  Runtime &r = rt();
+  std::lock_guard<std::recursive_mutex> lk(r.mu);
+  if (every < 0) return CLARITY_INVALID_ARGUMENT;
+  r.fault_every = every;
+  r.alloc_count = 0;
+  return CLARITY_SUCCESS;
+}
 
 }  // extern "C"

@@ -109,4 +109,10 @@ C_FUNC_DEF ClarityResult_t ClarityEx_Synchronize();
 /* Name of the GPU in use ("" before Clarity_Register). */
 C_FUNC_DEF const char *ClarityEx_DeviceName();
 
+/* Test hook: from now on every `every`-th device allocation fails with
+ * CLARITY_DEVICE_OUT_OF_MEMORY (1 = all of them, 0 = off).  The environment
+ * variable CLARITY_FAULT_DEVICE_OOM sets the same at Clarity_Register.  Lets
+ * callers check their out-of-memory handling without exhausting HBM. */
+C_FUNC_DEF ClarityResult_t ClarityEx_InjectDeviceOOM(int64_t every);
+
 #endif /* __CLARITY_EX_H_ */

@@ -1,0 +1,150 @@
+"""The fused, pruned power-of-two Richardson-Lucy engine ("fast").
+
+Parity against the CPU oracle (float64 FFT, fp64 reductions) at the small
+work shapes, against the generic engine at the headline shape, and the
+size-independent invariants of RL (energy, non-negativity, scale
+invariance) at 512x512x128.  Tolerances are the RL targets: rel L2 <= 1e-5
+after one iteration, <= 1e-4 after up to 100.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from clarity_amd import api as capi
+from clarity_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(autouse=True)
+def _fp64_oracle():
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+    yield
+
+
+def run_plan(api, obs, psf, iters, generic=False, initial=None):
+    if generic:
+        os.environ["CLARITY_FORCE_GENERIC"] = "1"
+    try:
+        plan = api.RLPlan(obs.shape, psf.shape)
+    finally:
+        os.environ.pop("CLARITY_FORCE_GENERIC", None)
+    engine = plan.engine
+    plan.set_inputs(obs, psf)
+    if initial is not None:
+        plan.reset(initial)
+    plan.iterate(iters)
+    out = plan.estimate()
+    plan.close()
+    return out, engine
+
+
+def _case(ishape, kshape, seed):
+    rng = np.random.default_rng(seed)
+    obj = synth.beads_and_filaments(ishape, max(10, int(np.prod(ishape) // 800)), 3, rng)
+    psf = synth.gaussian_kernel(kshape, 1.7, 3.2, side_lobe=0.1)
+    obs = synth.poisson(synth.np_convolve(obj, psf), rng)
+    return obs, psf
+
+
+SHAPES = [
+    ((32, 64, 64), (32, 64, 64)),      # config 1: work 64 x 128 x 128
+    ((30, 50, 61), (34, 78, 67)),      # odd image dims, work 64 x 128 x 128
+    ((64, 128, 128), (64, 128, 128)),  # work 128 x 256 x 256 (config-5 patch)
+    ((16, 100, 37), (16, 28, 91)),     # work 32? no: 32 x 128 x 128 -> generic z
+]
+
+
+@pytest.mark.parametrize("ishape,kshape", SHAPES[:3])
+def test_fast_vs_oracle(clarity_gpu, ishape, kshape):
+    api = clarity_gpu
+    obs, psf = _case(ishape, kshape, 21)
+    print(f"[fast] image {ishape} kernel {kshape}", flush=True)
+    for iters, tol in ((1, 1e-5), (8, 1e-4)):
+        got, engine = run_plan(api, obs, psf, iters)
+        assert engine == "fast"
+        ref = oracle.ml(obs, psf, iters)
+        err = rel_l2(got, ref)
+        print(f"[fast] iters {iters}: rel L2 {err:.3e}", flush=True)
+        assert err <= tol
+
+
+def test_fast_selected_only_for_supported_shapes(clarity_gpu):
+    api = clarity_gpu
+    p = api.RLPlan((32, 64, 64), (32, 64, 64))
+    assert p.engine == "fast"
+    p.close()
+    p = api.RLPlan((32, 64, 64), (31, 64, 64))   # nz = 63: generic
+    assert p.engine == "generic"
+    p.close()
+    p = api.RLPlan((16, 100, 37), (16, 28, 91))  # nz = 32 not in the fast set
+    assert p.engine == "generic"
+    p.close()
+
+
+def test_fast_matches_generic_100_iterations_cfg1(clarity_gpu):
+    api = clarity_gpu
+    case = synth.make_config(1)
+    a, ea = run_plan(api, case.obs, case.psf, 100)
+    b, eb = run_plan(api, case.obs, case.psf, 100, generic=True)
+    assert (ea, eb) == ("fast", "generic")
+    ref = oracle.ml(case.obs, case.psf, 100)
+    print(f"[fast] cfg1 100 it: fast {rel_l2(a, ref):.3e} generic {rel_l2(b, ref):.3e}")
+    assert rel_l2(a, ref) <= 1e-4
+    assert rel_l2(b, ref) <= 1e-4
+
+
+def test_fast_initial_estimate(clarity_gpu):
+    api = clarity_gpu
+    case = synth.make_config(1)
+    init = np.full(case.obs.shape, float(case.obs.mean()), np.float32)
+    got, engine = run_plan(api, case.obs, case.psf, 5, initial=init)
+    assert engine == "fast"
+    assert rel_l2(got, oracle.ml_ex(case.obs, case.psf, 5, initial=init)) <= 1e-4
+
+
+def test_fast_headline_shape_invariants(clarity_gpu):
+    """512x512x128 image, work 1024x1024x256: fast vs generic engine after
+    3 iterations, energy, non-negativity and scale invariance."""
+    api = clarity_gpu
+    shape = (128, 512, 512)
+    rng = np.random.default_rng(3)
+    obj = synth.beads_and_filaments(shape, 16000, 2400, rng)
+    psf = synth.gaussian_kernel(shape, 2.0, 4.5, side_lobe=0.1)
+    obs = synth.poisson(api.Convolve(obj, psf), rng)
+    a, ea = run_plan(api, obs, psf, 3)
+    b, eb = run_plan(api, obs, psf, 3, generic=True)
+    assert (ea, eb) == ("fast", "generic")
+    err = rel_l2(a, b)
+    print(f"[fast] 512x512x128, 3 it: fast vs generic rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-5
+    assert np.all(a >= 0)
+    assert abs(a.astype(np.float64).sum() / obs.astype(np.float64).sum() - 1) < 1e-5
+    c, _ = run_plan(api, 8 * obs, psf, 3)
+    assert rel_l2(c, 8 * a) <= 1e-5
+
+
+def test_fast_timing_api(clarity_gpu):
+    api = clarity_gpu
+    case = synth.make_config(1)
+    plan = api.RLPlan(case.obs.shape, case.psf.shape)
+    plan.set_inputs(case.obs, case.psf)
+    kinds = plan.describe()
+    assert [k[0] for k in kinds] == ["x_fwd", "y_fwd", "z_mod", "y_inv", "x_mid",
+                                     "x_last", "finish"]
+    plan.set_timing(True)
+    plan.timings(reset=True)
+    plan.iterate(3)
+    t = plan.timings()
+    assert [c for c, _ in t] == [3 * k[2] for k in kinds]
+    assert all(ms > 0 for _, ms in t)
+    plan.close()
