@@ -98,6 +98,9 @@ _SIGS = [
     ("ClarityEx_Synchronize", _R, []),
     ("ClarityEx_DeviceName", ctypes.c_char_p, []),
     ("ClarityEx_InjectDeviceOOM", _R, [ctypes.c_int64]),
+    ("ClarityEx_MaximumLikelihoodDeconvolveBatch", _R,
+     [ctypes.c_int, ctypes.POINTER(_F), ctypes.POINTER(Dim3), ctypes.POINTER(_F),
+      ctypes.POINTER(Dim3), ctypes.POINTER(_F), ctypes.c_int]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGS]
@@ -200,6 +203,23 @@ def MaximumLikelihoodDeconvolveFrom(img, kernel, initial, iterations: int):
         _p(img), dim3(img), _p(psf), dim3(psf), _p(init), _p(out), iterations),
         "ClarityEx_MaximumLikelihoodDeconvolveFrom")
     return out
+
+
+def MaximumLikelihoodDeconvolveBatch(images, kernels, iterations: int):
+    """ClarityEx_MaximumLikelihoodDeconvolveBatch over lists of volumes."""
+    imgs = [_vol(a) for a in images]
+    ks = [_vol(k) for k in kernels]
+    if len(imgs) != len(ks):
+        raise ValueError("one kernel per image")
+    outs = [np.empty_like(a) for a in imgs]
+    n = len(imgs)
+    P = _F * n
+    D = Dim3 * n
+    _check(lib().ClarityEx_MaximumLikelihoodDeconvolveBatch(
+        n, P(*[_p(a) for a in imgs]), D(*[dim3(a) for a in imgs]),
+        P(*[_p(k) for k in ks]), D(*[dim3(k) for k in ks]),
+        P(*[_p(o) for o in outs]), iterations), "ClarityEx_MaximumLikelihoodDeconvolveBatch")
+    return outs
 
 
 def BlindRL(img, kernel, outer: int, inner: int):

@@ -182,6 +182,72 @@ ClarityResult_t ml_call(float *in, Clarity_Dim3 id, float *kernel,
   return plan_get(p, out, 0);
 }
 
+// Independent volumes on a pool of streams: volume v runs on stream
+// v % nstreams, so up to nstreams volumes' kernels overlap on the GPU.  Each
+// volume is computed exactly as ml_call would compute it alone.
+ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
+                         float *const *kers, const Clarity_Dim3 *kds,
+                         float *const *outs, int iterations) {
+  if (count < 0 || iterations < 0) return CLARITY_INVALID_ARGUMENT;
+  if (count == 0) return CLARITY_SUCCESS;
+  if (!ins || !ids || !kers || !kds || !outs) return CLARITY_INVALID_ARGUMENT;
+  for (int v = 0; v < count; v++)
+    if (!ins[v] || !kers[v] || !outs[v] || !dims_ok(ids[v]) || !dims_ok(kds[v]))
+      return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  struct Vol {
+    std::unique_ptr<ClarityEx_RLPlan_t> plan;
+    DevBuf a, b;
+  };
+  const int nstreams = count < 8 ? count : 8;
+  std::vector<hipStream_t> streams(nstreams, nullptr);
+  ClarityResult_t rc = CLARITY_SUCCESS;
+  for (int i = 0; i < nstreams && rc == CLARITY_SUCCESS; i++)
+    rc = from_hip(hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking));
+  std::vector<Vol> vols(count);
+  for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++) {
+    hipStream_t s = streams[v % nstreams];
+    ClarityEx_RLPlan p = nullptr;
+    rc = plan_create(&p, ids[v], kds[v]);
+    if (rc != CLARITY_SUCCESS) break;
+    vols[v].plan.reset(p);
+    const Work &w = p->geom;
+    rc = vols[v].a.alloc(sizeof(float) * w.IV);
+    if (rc == CLARITY_SUCCESS) rc = vols[v].b.alloc(sizeof(float) * w.KV);
+    if (rc == CLARITY_SUCCESS)
+      rc = from_hip(hipMemcpyAsync(vols[v].a.p, ins[v], sizeof(float) * w.IV,
+                                   hipMemcpyHostToDevice, s));
+    if (rc == CLARITY_SUCCESS)
+      rc = from_hip(hipMemcpyAsync(vols[v].b.p, kers[v], sizeof(float) * w.KV,
+                                   hipMemcpyHostToDevice, s));
+    if (rc == CLARITY_SUCCESS) rc = p->eng->set_inputs(vols[v].a.f(), vols[v].b.f(), s);
+    if (rc == CLARITY_SUCCESS) rc = p->eng->reset(nullptr, s);
+    p->have_inputs = true;
+  }
+  // interleave: one iteration of every volume at a time, so all streams stay
+  // busy until the end
+  for (int it = 0; it < iterations && rc == CLARITY_SUCCESS; it++)
+    for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++)
+      rc = vols[v].plan->eng->iterate(1, streams[v % nstreams]);
+  for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++) {
+    hipStream_t s = streams[v % nstreams];
+    const Work &w = vols[v].plan->geom;
+    rc = vols[v].plan->eng->estimate(vols[v].a.f(), s);
+    if (rc == CLARITY_SUCCESS)
+      rc = from_hip(hipMemcpyAsync(outs[v], vols[v].a.p, sizeof(float) * w.IV,
+                                   hipMemcpyDeviceToHost, s));
+  }
+  for (hipStream_t s : streams) {
+    if (!s) continue;
+    ClarityResult_t r2 = from_hip(hipStreamSynchronize(s));
+    if (rc == CLARITY_SUCCESS) rc = r2;
+  }
+  vols.clear();  // frees device buffers while the streams still exist
+  for (hipStream_t s : streams)
+    if (s) (void)hipStreamDestroy(s);
+  return rc;
+}
+
 }  // namespace
 
 #define API_LOCK std::lock_guard<std::recursive_mutex> lk_(rt().mu)
@@ -203,6 +269,15 @@ ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveFrom(
   API_LOCK;
   return ml_call(inImage, imageDim, kernelImage, kernelDim, initialImage,
                  outImage, iterations);
+}
+
+ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveBatch(
+    int count, float *const *inImages, const Clarity_Dim3 *imageDims,
+    float *const *kernelImages, const Clarity_Dim3 *kernelDims,
+    float *const *outImages, int iterations) {
+  API_LOCK;
+  return ml_batch(count, inImages, imageDims, kernelImages, kernelDims,
+                  outImages, iterations);
 }
 
 ClarityResult_t ClarityEx_RLPlanCreate(ClarityEx_RLPlan *plan,

@@ -45,6 +45,16 @@ C_FUNC_DEF ClarityResult_t ClarityEx_IDivergence(const float *g,
                                                  const float *ghat, int64_t n,
                                                  double *result);
 
+/* Richardson-Lucy on `count` independent volumes of (possibly) different
+ * sizes, each with its own kernel, `iterations` updates each; every volume
+ * gets exactly the result of Clarity_MaximumLikelihoodDeconvolve on it.  The
+ * volumes run concurrently on several HIP streams (small volumes alone do
+ * not fill an MI355X).  Arrays of host pointers and dims, `count` long. */
+C_FUNC_DEF ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveBatch(
+    int count, float *const *inImages, const Clarity_Dim3 *imageDims,
+    float *const *kernelImages, const Clarity_Dim3 *kernelDims,
+    float *const *outImages, int iterations);
+
 /* ---- resident Richardson-Lucy plan ------------------------------------ */
 
 typedef struct ClarityEx_RLPlan_t *ClarityEx_RLPlan;

@@ -148,3 +148,22 @@ def test_fast_timing_api(clarity_gpu):
     assert [c for c, _ in t] == [3 * k[2] for k in kinds]
     assert all(ms > 0 for _, ms in t)
     plan.close()
+
+
+def test_batch_uneven_volumes(clarity_gpu):
+    """Batched RL over uneven volumes equals per-volume RL (each volume's
+    result is the single-volume call's, to float rounding of the engine)."""
+    api = clarity_gpu
+    rng = np.random.default_rng(44)
+    shapes = [(32, 64, 64), (16, 48, 40), (24, 96, 64), (32, 64, 64), (8, 20, 30)]
+    imgs, ks = [], []
+    for i, s in enumerate(shapes):
+        obj = synth.beads_and_filaments(s, 30, 2, rng)
+        k = synth.gaussian_kernel(s, rng.uniform(1.2, 2.5), rng.uniform(2.5, 5.0))
+        imgs.append(synth.poisson(synth.np_convolve(obj, k), rng))
+        ks.append(k)
+    outs = api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 6)
+    for a, k, o in zip(imgs, ks, outs):
+        single = api.MaximumLikelihoodDeconvolve(a, k, 6)
+        assert rel_l2(o, single) <= 1e-6
+        assert rel_l2(o, oracle.ml(a, k, 6)) <= 1e-4
