@@ -1,4 +1,5 @@
-// Fused, pruned Richardson-Lucy pipeline for power-of-two work shapes.
+// Fused, pruned Richardson-Lucy pipeline for work shapes whose axes are
+// 2^k or 3 * 2^k (x 128..1024, y 128..2048, z 64..512).
 //
 // The RL estimate and the ratio obs/s1 are zero outside the image corner of
 // the work domain (the estimate starts as the zero-padded observation and
@@ -95,6 +96,37 @@ __device__ __forceinline__ void dft16(float2 (&u)[16]) {
   }
 }
 
+// 3-point DFT: X1,2 = a - (b+c)/2 +- SIGN*i*(sqrt(3)/2)*(b-c)
+template <int SIGN>
+__device__ __forceinline__ void dft3(float2 &a, float2 &b, float2 &c) {
+  constexpr float h = 0.86602540378443864676f;
+  const float2 t1 = cadd(b, c);
+  const float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+  const float2 d = csub(b, c);
+  const float2 r = make_float2(-SIGN * h * d.y, SIGN * h * d.x);  // SIGN*i*h*d
+  a = cadd(a, t1);
+  b = cadd(t2, r);
+  c = csub(t2, r);
+}
+
+// 6-point DFT: n = 3*n1 + n2, k = k1 + 2*k2; three DFT2 over n1, twiddle
+// w6^(n2*k1), two DFT3 over n2.
+template <int SIGN>
+__device__ __forceinline__ void dft6(float2 (&u)[6]) {
+  constexpr float h = 0.86602540378443864676f;
+  float2 a0 = u[0], b0 = u[3], a1 = u[1], b1 = u[4], a2 = u[2], b2 = u[5];
+  dft2(a0, b0);
+  dft2(a1, b1);
+  dft2(a2, b2);
+  // w6^1 = 1/2 + SIGN*i*h, w6^2 = -1/2 + SIGN*i*h
+  b1 = make_float2(0.5f * b1.x - SIGN * h * b1.y, SIGN * h * b1.x + 0.5f * b1.y);
+  b2 = make_float2(-0.5f * b2.x - SIGN * h * b2.y, SIGN * h * b2.x - 0.5f * b2.y);
+  dft3<SIGN>(a0, a1, a2);
+  dft3<SIGN>(b0, b1, b2);
+  u[0] = a0; u[2] = a1; u[4] = a2;
+  u[1] = b0; u[3] = b1; u[5] = b2;
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft(float2 (&u)[R]) {
   if constexpr (R == 1) {
@@ -106,6 +138,10 @@ __device__ __forceinline__ void dft(float2 (&u)[R]) {
     dft8(u, SIGN);
   } else if constexpr (R == 16) {
     dft16<SIGN>(u);
+  } else if constexpr (R == 3) {
+    dft3<SIGN>(u[0], u[1], u[2]);
+  } else if constexpr (R == 6) {
+    dft6<SIGN>(u);
   } else {
     static_assert(R == 16, "unsupported radix");
   }
@@ -140,21 +176,42 @@ struct PencilLds {
 
 // ------------------------------------------------------------ FFT engine
 
-// Stockham stages from span P on.  Thread t of a pencil holds x[t + T*e] in
-// v[e] on entry; on exit v[e] holds X[t + T*e] (natural order).  All threads
-// of the workgroup must call this together (it synchronises).
-template <int N, int E, int P, int SIGN, class L>
+// Compile-time radix sequences.  The default for a power of two is E, E,
+// ..., then the remainder; mixed sizes list theirs explicitly.
+template <int... Rs>
+struct Radices {};
+
+template <int N, int E, int P, bool Done, int... Acc>
+struct RadixList;
+template <int N, int E, int P, int... Acc>
+struct RadixList<N, E, P, true, Acc...> {
+  using type = Radices<Acc...>;
+};
+template <int N, int E, int P, int... Acc>
+struct RadixList<N, E, P, false, Acc...> {
+  static constexpr int R = (N / P >= E) ? E : N / P;
+  using type = typename RadixList<N, E, P * R, (P * R == N), Acc..., R>::type;
+};
+template <int N, int E>
+using DefaultRadices = typename RadixList<N, E, 1, false>::type;
+
+// Stockham stages from span P on, one per radix of the list.  Thread t of a
+// pencil holds x[t + T*e] in v[e] on entry; on exit v[e] holds X[t + T*e]
+// (natural order).  All threads of the workgroup must call this together
+// (it synchronises).
+template <int N, int E, int P, int SIGN, class L, int R, int... Rest>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
-                                           const float2 *__restrict__ tw) {
+                                           const float2 *__restrict__ tw,
+                                           Radices<R, Rest...>) {
   constexpr int T = N / E;
-  constexpr int R = (N / P >= E) ? E : N / P;
-  constexpr bool LAST = (P * R == N);
+  constexpr bool LAST = sizeof...(Rest) == 0;
   constexpr int NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread count");
+  static_assert(!LAST || P * R == N, "radices must multiply to N");
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     const int i = t + T * b;
-    const int k = i & (P - 1);
+    const int k = i % P;
     float2 u[R];
 #pragma unroll
     for (int r = 0; r < R; r++) u[r] = v[b + r * NB];
@@ -178,9 +235,42 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = lds(t + T * e);
     __syncthreads();
-    fft_stages<N, E, P * R, SIGN, L>(v, t, lds, tw);
+    fft_stages<N, E, P * R, SIGN, L>(v, t, lds, tw, Radices<Rest...>{});
   }
 }
+
+// Per-size transform shapes: E elements per thread, radix list, and for the
+// y pass PPW pencils per workgroup.  Powers of two use radix E (8 or 16)
+// stages; 3 * 2^k sizes use E = 12 with radix 3 or 6 first.
+template <int N>
+struct XCfg;  // x pass, N = nx/2 (half-length complex transform)
+template <> struct XCfg<64> { static constexpr int E = 8; using Rad = DefaultRadices<64, 8>; };
+template <> struct XCfg<96> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 2>; };
+template <> struct XCfg<128> { static constexpr int E = 8; using Rad = DefaultRadices<128, 8>; };
+template <> struct XCfg<192> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 4>; };
+template <> struct XCfg<256> { static constexpr int E = 16; using Rad = DefaultRadices<256, 16>; };
+template <> struct XCfg<384> { static constexpr int E = 12; using Rad = Radices<6, 4, 4, 4>; };
+template <> struct XCfg<512> { static constexpr int E = 8; using Rad = DefaultRadices<512, 8>; };
+
+template <int N>
+struct YCfg;
+template <> struct YCfg<128> { static constexpr int E = 8, PPW = 16; using Rad = DefaultRadices<128, 8>; };
+template <> struct YCfg<192> { static constexpr int E = 12, PPW = 16; using Rad = Radices<3, 4, 4, 4>; };
+template <> struct YCfg<256> { static constexpr int E = 16, PPW = 16; using Rad = DefaultRadices<256, 16>; };
+template <> struct YCfg<384> { static constexpr int E = 12, PPW = 8; using Rad = Radices<6, 4, 4, 4>; };
+template <> struct YCfg<512> { static constexpr int E = 8, PPW = 4; using Rad = DefaultRadices<512, 8>; };
+template <> struct YCfg<768> { static constexpr int E = 12, PPW = 4; using Rad = Radices<3, 4, 4, 4, 4>; };
+template <> struct YCfg<1024> { static constexpr int E = 16, PPW = 4; using Rad = DefaultRadices<1024, 16>; };
+template <> struct YCfg<2048> { static constexpr int E = 16, PPW = 2; using Rad = DefaultRadices<2048, 16>; };
+
+template <int N>
+struct ZCfg;
+template <> struct ZCfg<64> { static constexpr int E = 8; using Rad = DefaultRadices<64, 8>; };
+template <> struct ZCfg<96> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 2>; };
+template <> struct ZCfg<128> { static constexpr int E = 8; using Rad = DefaultRadices<128, 8>; };
+template <> struct ZCfg<192> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 4>; };
+template <> struct ZCfg<256> { static constexpr int E = 16; using Rad = DefaultRadices<256, 16>; };
+template <> struct ZCfg<512> { static constexpr int E = 16; using Rad = DefaultRadices<512, 16>; };
 
 struct Geom {
   int ix, iy, iz;  // image
@@ -206,8 +296,10 @@ __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restri
     const int qq = idx % TP;
     const int k = idx / TP;
     const int sw = qq & 15;
-    const float2 zk = lds[qq * M + ((k & (M - 1)) ^ sw)];
-    const float2 zm = lds[qq * M + (((M - k) & (M - 1)) ^ sw)];
+    const int kk = (k == M) ? 0 : k;       // Z is M-periodic
+    const int km = (k == 0) ? 0 : M - k;
+    const float2 zk = lds[qq * M + (kk ^ sw)];
+    const float2 zm = lds[qq * M + (km ^ sw)];
     const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);   // Zk + conj(Zm)
     const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);   // Zk - conj(Zm)
     const float2 wd = cmul(twN[k], d);                         // w^k d
@@ -265,7 +357,7 @@ __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const RowLds &L, in
   }
 }
 
-template <int M, int E, int TP>
+template <int M, int TP, int E = XCfg<M>::E>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
             const float2 *__restrict__ twM, const float2 *__restrict__ twN, Geom g) {
@@ -288,7 +380,7 @@ __global__ void __launch_bounds__(TP *(M / E))
     v[e] = make_float2(a, b);
   }
   const RowLds L{lds + q * M, q & 15};
-  fft_stages<M, E, 1, -1>(v, t, L, twM);
+  fft_stages<M, E, 1, -1>(v, t, L, twM, typename XCfg<M>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
@@ -307,10 +399,10 @@ __device__ __forceinline__ void inverse_rows(float2 (&v)[E], float2 *lds, const 
   __syncthreads();
   pack_inverse<M, E>(v, L, t, twN);
   __syncthreads();
-  fft_stages<M, E, 1, +1>(v, t, L, twM);
+  fft_stages<M, E, 1, +1>(v, t, L, twM, typename XCfg<M>::Rad{});
 }
 
-template <int M, int E, int TP>
+template <int M, int TP, int E = XCfg<M>::E>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
             const float *__restrict__ cptr, const float2 *__restrict__ twM,
@@ -346,14 +438,14 @@ __global__ void __launch_bounds__(TP *(M / E))
     }
     v[e] = make_float2(r0, r1);
   }
-  fft_stages<M, E, 1, -1>(v, t, L, twM);
+  fft_stages<M, E, 1, -1>(v, t, L, twM, typename XCfg<M>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
   unpack_store<M, TP, NT>(lds, S, twN, g, z, y0);
 }
 
-template <int M, int E, int TP>
+template <int M, int TP, int E = XCfg<M>::E>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
@@ -404,7 +496,7 @@ __global__ void __launch_bounds__(TP *(M / E))
 
 // ------------------------------------------------------------- y and z
 
-template <int N, int E, int PPW, int SIGN>
+template <int N, int SIGN, int E = YCfg<N>::E, int PPW = YCfg<N>::PPW>
 __global__ void __launch_bounds__(PPW *(N / E))
     k_y(float2 *__restrict__ S, const float2 *__restrict__ tw, long long npencils,
         int Lin, int Lout) {
@@ -421,7 +513,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
     v[e] = (valid && j < Lin) ? base[j] : make_float2(0.f, 0.f);
   }
   const PencilLds L{lds + pl * N};
-  fft_stages<N, E, 1, SIGN>(v, t, L, tw);
+  fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename YCfg<N>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int j = t + T * e;
@@ -429,7 +521,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
 }
 
-template <int N, int E, int TP>
+template <int N, int TP, int E = ZCfg<N>::E>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
         const float2 *__restrict__ tw, Geom g, float scale) {
@@ -447,7 +539,7 @@ __global__ void __launch_bounds__(TP *(N / E))
     v[e] = (z < g.iz) ? S[z * g.sz + col] : make_float2(0.f, 0.f);
   }
   const InterLds<TP> L{lds + q};
-  fft_stages<N, E, 1, -1>(v, t, L, tw);
+  fft_stages<N, E, 1, -1>(v, t, L, tw, typename ZCfg<N>::Rad{});
   // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -456,7 +548,7 @@ __global__ void __launch_bounds__(TP *(N / E))
     const float2 a = v[e];
     v[e] = make_float2(scale * (a.x * h.x - a.y * h.y), scale * (a.x * h.y + a.y * h.x));
   }
-  fft_stages<N, E, 1, +1>(v, t, L, tw);
+  fft_stages<N, E, 1, +1>(v, t, L, tw, typename ZCfg<N>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int z = t + T * e;
@@ -477,33 +569,16 @@ struct Launch {
   unsigned grid = 0, block = 0;
 };
 
-template <int M>
-struct XCfg;
-template <> struct XCfg<64> { static constexpr int E = 8; };
-template <> struct XCfg<128> { static constexpr int E = 8; };
-template <> struct XCfg<256> { static constexpr int E = 16; };
-template <> struct XCfg<512> { static constexpr int E = 8; };
-
-template <int N>
-struct YCfg;
-template <> struct YCfg<128> { static constexpr int E = 8, PPW = 16; };
-template <> struct YCfg<256> { static constexpr int E = 16, PPW = 16; };
-template <> struct YCfg<512> { static constexpr int E = 8, PPW = 4; };
-template <> struct YCfg<1024> { static constexpr int E = 16, PPW = 4; };
-template <> struct YCfg<2048> { static constexpr int E = 16, PPW = 2; };
-
-template <int N>
-struct ZCfg;
-template <> struct ZCfg<64> { static constexpr int E = 8; };
-template <> struct ZCfg<128> { static constexpr int E = 8; };
-template <> struct ZCfg<256> { static constexpr int E = 16; };
-template <> struct ZCfg<512> { static constexpr int E = 16; };
-
-bool x_supported(int M) { return M == 64 || M == 128 || M == 256 || M == 512; }
-bool y_supported(int N) {
-  return N == 128 || N == 256 || N == 512 || N == 1024 || N == 2048;
+bool x_supported(int M) {
+  return M == 64 || M == 96 || M == 128 || M == 192 || M == 256 || M == 384 || M == 512;
 }
-bool z_supported(int N) { return N == 64 || N == 128 || N == 256 || N == 512; }
+bool y_supported(int N) {
+  return N == 128 || N == 192 || N == 256 || N == 384 || N == 512 || N == 768 ||
+         N == 1024 || N == 2048;
+}
+bool z_supported(int N) {
+  return N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 512;
+}
 
 struct Bufs {
   float *u, *obs, *scal;  // scal: E, S, c
@@ -514,24 +589,26 @@ struct Bufs {
 
 template <int M>
 void launch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  constexpr int E = XCfg<M>::E;
-  constexpr int NT = kTPx * (M / E);
+  constexpr int NT = kTPx * (M / XCfg<M>::E);
   const unsigned grid = (unsigned)g.iz * ((g.iy + kTPx - 1) / kTPx);
   if (kind == KX_FWD)
-    hipLaunchKernelGGL((k_x_fwd<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+    hipLaunchKernelGGL((k_x_fwd<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
   else if (kind == KX_MID)
-    hipLaunchKernelGGL((k_x_mid<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+    hipLaunchKernelGGL((k_x_mid<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
   else
-    hipLaunchKernelGGL((k_x_last<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
 }
 
 void dispatch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   switch (g.nx / 2) {
     case 64: launch_x<64>(kind, g, b, s); break;
+    case 96: launch_x<96>(kind, g, b, s); break;
     case 128: launch_x<128>(kind, g, b, s); break;
+    case 192: launch_x<192>(kind, g, b, s); break;
+    case 384: launch_x<384>(kind, g, b, s); break;
     case 256: launch_x<256>(kind, g, b, s); break;
     case 512: launch_x<512>(kind, g, b, s); break;
   }
@@ -543,16 +620,19 @@ void launch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   const long long np = (long long)g.iz * g.hx;
   const unsigned grid = (unsigned)((np + PPW - 1) / PPW);
   if (fwd)
-    hipLaunchKernelGGL((k_y<N, E, PPW, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+    hipLaunchKernelGGL((k_y<N, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
                        b.twY, np, g.iy, N);
   else
-    hipLaunchKernelGGL((k_y<N, E, PPW, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+    hipLaunchKernelGGL((k_y<N, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
                        b.twY, np, N, g.iy);
 }
 
 void dispatch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   switch (g.ny) {
     case 128: launch_y<128>(fwd, g, b, s); break;
+    case 192: launch_y<192>(fwd, g, b, s); break;
+    case 384: launch_y<384>(fwd, g, b, s); break;
+    case 768: launch_y<768>(fwd, g, b, s); break;
     case 256: launch_y<256>(fwd, g, b, s); break;
     case 512: launch_y<512>(fwd, g, b, s); break;
     case 1024: launch_y<1024>(fwd, g, b, s); break;
@@ -564,20 +644,20 @@ template <int N>
 void launch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   constexpr int E = ZCfg<N>::E;
   const unsigned grid = (unsigned)(g.hx * (g.ny / kTPz));
-  hipLaunchKernelGGL((k_z<N, E, kTPz>), dim3(grid), dim3(kTPz * (N / E)), 0, s, b.S,
+  hipLaunchKernelGGL((k_z<N, kTPz>), dim3(grid), dim3(kTPz * (N / E)), 0, s, b.S,
                      b.otf, b.twZ, g, scale);
 }
 
 void dispatch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   switch (g.nz) {
     case 64: launch_z<64>(g, b, scale, s); break;
+    case 96: launch_z<96>(g, b, scale, s); break;
     case 128: launch_z<128>(g, b, scale, s); break;
+    case 192: launch_z<192>(g, b, scale, s); break;
     case 256: launch_z<256>(g, b, scale, s); break;
     case 512: launch_z<512>(g, b, scale, s); break;
   }
 }
-
-bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // --------------------------------------------------------------- engine
 
@@ -599,8 +679,8 @@ class FastRL : public RLEngine {
   }
 
   static bool supports(const Work &w) {
-    return pow2(w.wd.x) && pow2(w.wd.y) && pow2(w.wd.z) && x_supported(w.wd.x / 2) &&
-           y_supported(w.wd.y) && z_supported(w.wd.z) && w.wd.y >= kTPz;
+    return w.wd.x % 2 == 0 && x_supported(w.wd.x / 2) && y_supported(w.wd.y) &&
+           z_supported(w.wd.z) && w.wd.y % kTPz == 0;
   }
 
   ClarityResult_t allocate() override {

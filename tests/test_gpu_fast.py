@@ -60,11 +60,13 @@ SHAPES = [
     ((32, 64, 64), (32, 64, 64)),      # config 1: work 64 x 128 x 128
     ((30, 50, 61), (34, 78, 67)),      # odd image dims, work 64 x 128 x 128
     ((64, 128, 128), (64, 128, 128)),  # work 128 x 256 x 256 (config-5 patch)
-    ((16, 100, 37), (16, 28, 91)),     # work 32? no: 32 x 128 x 128 -> generic z
+    ((48, 96, 96), (48, 96, 96)),      # work 96 x 192 x 192: radix 3
+    ((32, 192, 64), (32, 192, 64)),    # work 64 x 384 x 128: radix 6 in y
+    ((40, 70, 150), (56, 122, 42)),    # work 96 x 192 x 192 from odd dims
 ]
 
 
-@pytest.mark.parametrize("ishape,kshape", SHAPES[:3])
+@pytest.mark.parametrize("ishape,kshape", SHAPES)
 def test_fast_vs_oracle(clarity_gpu, ishape, kshape):
     api = clarity_gpu
     obs, psf = _case(ishape, kshape, 21)
