@@ -70,6 +70,11 @@ class Hip:
         if rc != 0:
             raise RuntimeError(f"hipMemcpy D2H failed: {rc}")
 
+    def d2d(self, dst, src, n):
+        rc = self.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), n, 3)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy D2D failed: {rc}")
+
     def free(self, p):
         self.L.hipFree(ctypes.c_void_p(p))
 
@@ -115,6 +120,10 @@ def main():
     ap.add_argument("--shape", type=str, default="128,512,512", help="image z,y,x")
     ap.add_argument("--cpu-iterations", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device 3 times "
+                         "(a plain copy kernel of known size, to calibrate rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -231,6 +240,14 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_mib:
+        n = args.copy_mib << 20
+        src, dst = hip.malloc(n), hip.malloc(n)
+        for _ in range(3):
+            hip.d2d(dst, src, n)
+        hip.free(src)
+        hip.free(dst)
+        log(f"[bench] copied {args.copy_mib} MiB device-to-device 3 times")
     plan.close()
     for p in (d_obs, d_psf, d_out):
         hip.free(p)

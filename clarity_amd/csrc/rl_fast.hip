@@ -28,6 +28,7 @@
 #include "rl_plan.hpp"
 
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace clarity {
@@ -239,49 +240,34 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
   }
 }
 
-// Per-size transform shapes: E elements per thread, radix list, and for the
-// y pass PPW pencils per workgroup.  Powers of two use radix E (8 or 16)
-// stages; 3 * 2^k sizes use E = 12 with radix 3 or 6 first.
-template <int N>
-struct XCfg;  // x pass, N = nx/2 (half-length complex transform)
-template <> struct XCfg<64> { static constexpr int E = 8; using Rad = DefaultRadices<64, 8>; };
-template <> struct XCfg<96> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 2>; };
-template <> struct XCfg<128> { static constexpr int E = 8; using Rad = DefaultRadices<128, 8>; };
-template <> struct XCfg<192> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 4>; };
-template <> struct XCfg<256> { static constexpr int E = 16; using Rad = DefaultRadices<256, 16>; };
-template <> struct XCfg<384> { static constexpr int E = 12; using Rad = Radices<6, 4, 4, 4>; };
-template <> struct XCfg<512> { static constexpr int E = 8; using Rad = DefaultRadices<512, 8>; };
-
-template <int N>
-struct YCfg;
-template <> struct YCfg<128> { static constexpr int E = 8, PPW = 16; using Rad = DefaultRadices<128, 8>; };
-template <> struct YCfg<192> { static constexpr int E = 12, PPW = 16; using Rad = Radices<3, 4, 4, 4>; };
-template <> struct YCfg<256> { static constexpr int E = 16, PPW = 16; using Rad = DefaultRadices<256, 16>; };
-template <> struct YCfg<384> { static constexpr int E = 12, PPW = 8; using Rad = Radices<6, 4, 4, 4>; };
-template <> struct YCfg<512> { static constexpr int E = 8, PPW = 4; using Rad = DefaultRadices<512, 8>; };
-template <> struct YCfg<768> { static constexpr int E = 12, PPW = 4; using Rad = Radices<3, 4, 4, 4, 4>; };
-template <> struct YCfg<1024> { static constexpr int E = 16, PPW = 4; using Rad = DefaultRadices<1024, 16>; };
-template <> struct YCfg<2048> { static constexpr int E = 16, PPW = 2; using Rad = DefaultRadices<2048, 16>; };
-
-template <int N>
-struct ZCfg;
-template <> struct ZCfg<64> { static constexpr int E = 8; using Rad = DefaultRadices<64, 8>; };
-template <> struct ZCfg<96> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 2>; };
-template <> struct ZCfg<128> { static constexpr int E = 8; using Rad = DefaultRadices<128, 8>; };
-template <> struct ZCfg<192> { static constexpr int E = 12; using Rad = Radices<3, 4, 4, 4>; };
-template <> struct ZCfg<256> { static constexpr int E = 16; using Rad = DefaultRadices<256, 16>; };
-template <> struct ZCfg<512> { static constexpr int E = 16; using Rad = DefaultRadices<512, 16>; };
+// Radix plan for an N-point transform with E elements per thread: powers of
+// two use radix-E stages then the remainder; 3 * 2^k sizes use E = 12 with a
+// radix 3 or 6 stage first.
+template <int N, int E>
+struct Plan {
+  using Rad = DefaultRadices<N, E>;
+};
+template <> struct Plan<96, 12> { using Rad = Radices<3, 4, 4, 2>; };
+template <> struct Plan<192, 12> { using Rad = Radices<3, 4, 4, 4>; };
+template <> struct Plan<384, 12> { using Rad = Radices<6, 4, 4, 4>; };
+template <> struct Plan<768, 12> { using Rad = Radices<3, 4, 4, 4, 4>; };
 
 struct Geom {
   int ix, iy, iz;  // image
-  int nx, ny, nz;  // work (powers of two)
+  int nx, ny, nz;  // work
   int hx;          // nx/2 + 1
   long long sz;    // S / OTF stride of z: hx*ny
 };
 
+// Copies a twiddle table into LDS (cooperatively; caller synchronises).
+template <int NT>
+__device__ __forceinline__ void stage_table(float2 *dst, const float2 *__restrict__ src,
+                                            int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------- x passes
 
-template <int M>
 __device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y) {
   return ((long long)z * g.hx + k) * g.ny + y;
 }
@@ -290,22 +276,22 @@ __device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y)
 // LDS, to S with consecutive rows on consecutive lanes.
 template <int M, int TP, int NT>
 __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restrict__ S,
-                                             const float2 *__restrict__ twN,
-                                             const Geom &g, int z, int y0) {
+                                             const float2 *twN, const Geom &g, int z,
+                                             int y0) {
   for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
     const int qq = idx % TP;
     const int k = idx / TP;
     const int sw = qq & 15;
-    const int kk = (k == M) ? 0 : k;       // Z is M-periodic
+    const int kk = (k == M) ? 0 : k;  // Z is M-periodic
     const int km = (k == 0) ? 0 : M - k;
     const float2 zk = lds[qq * M + (kk ^ sw)];
     const float2 zm = lds[qq * M + (km ^ sw)];
-    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);   // Zk + conj(Zm)
-    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);   // Zk - conj(Zm)
-    const float2 wd = cmul(twN[k], d);                         // w^k d
+    const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);  // Zk + conj(Zm)
+    const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Zk - conj(Zm)
+    const float2 wd = cmul(twN[k], d);                        // w^k d
     // X = (a - i w^k d) / 2
     const float2 X = make_float2(0.5f * (a.x + wd.y), 0.5f * (a.y - wd.x));
-    if (y0 + qq < g.iy) S[s_index<M>(g, z, k, y0 + qq)] = X;
+    if (y0 + qq < g.iy) S[s_index(g, z, k, y0 + qq)] = X;
   }
 }
 
@@ -319,7 +305,7 @@ __device__ __forceinline__ void load_spectrum(float2 *lds, const float2 *__restr
     const int qq = idx % TP;
     const int k = idx / TP;
     float2 X = make_float2(0.f, 0.f);
-    if (y0 + qq < g.iy) X = S[s_index<M>(g, z, k, y0 + qq)];
+    if (y0 + qq < g.iy) X = S[s_index(g, z, k, y0 + qq)];
     const int sw = qq & 15;
     if (k == 0)
       lds[qq * M + sw].x = X.x;
@@ -334,7 +320,7 @@ __device__ __forceinline__ void load_spectrum(float2 *lds, const float2 *__restr
 // thread's k = t + T*e; its inverse FFT is x[2m] + i x[2m+1].
 template <int M, int E>
 __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const RowLds &L, int t,
-                                             const float2 *__restrict__ twN) {
+                                             const float2 *twN) {
   constexpr int T = M / E;
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -351,139 +337,156 @@ __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const RowLds &L, in
     const float2 a = make_float2(xk.x + xm.x, xk.y - xm.y);
     const float2 d = make_float2(xk.x - xm.x, xk.y + xm.y);
     float2 w = twN[k];
-    w.y = -w.y;                                  // w^-k
+    w.y = -w.y;  // w^-k
     const float2 wd = cmul(w, d);
     v[e] = make_float2(a.x - wd.y, a.y + wd.x);  // a + i w^-k d
   }
 }
 
-template <int M, int TP, int E = XCfg<M>::E>
-__global__ void __launch_bounds__(TP *(M / E))
-    k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
-            const float2 *__restrict__ twM, const float2 *__restrict__ twN, Geom g) {
-  constexpr int T = M / E;
-  constexpr int NT = TP * T;
-  __shared__ float2 lds[TP * M];
-  const int q = threadIdx.x / T, t = threadIdx.x % T;
-  const int ntile = (g.iy + TP - 1) / TP;
-  const int z = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
-  const int y = y0 + q;
-  const bool valid = y < g.iy;
-  const float *row = u + ((long long)z * g.iy + y) * g.ix;
-  float2 v[E];
+// Row loads of a real image row pair (x[2m], x[2m+1]) for m = t + T*e.
+template <int E, int T>
+__device__ __forceinline__ void load_row_pairs(float2 (&v)[E], const float *row, bool valid,
+                                               int t, int ix) {
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
-    const float a = (valid && n0 < g.ix) ? row[n0] : 0.f;
-    const float b = (valid && n0 + 1 < g.ix) ? row[n0 + 1] : 0.f;
+    const float a = (valid && n0 < ix) ? row[n0] : 0.f;
+    const float b = (valid && n0 + 1 < ix) ? row[n0 + 1] : 0.f;
     v[e] = make_float2(a, b);
   }
-  const RowLds L{lds + q * M, q & 15};
-  fft_stages<M, E, 1, -1>(v, t, L, twM, typename XCfg<M>::Rad{});
+}
+
+// x kernels share one shared-memory block: the TP-row tile, then the M-point
+// and (M+1)-entry twiddle tables.
+template <int M, int TP>
+struct XSmem {
+  float2 tile[TP * M];
+  float2 twM[M];
+  float2 twN[M + 1];
+};
+
+template <int M, int E, int TP>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
+            const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  constexpr int T = M / E;
+  constexpr int NT = TP * T;
+  __shared__ XSmem<M, TP> sm;
+  const int q = threadIdx.x / T, t = threadIdx.x % T;
+  const int ntile = (g.iy + TP - 1) / TP;
+  const int z = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const int y = y0 + q;
+  float2 v[E];
+  load_row_pairs<E, T>(v, u + ((long long)z * g.iy + y) * g.ix, y < g.iy, t, g.ix);
+  stage_table<NT>(sm.twM, gtwM, M);
+  stage_table<NT>(sm.twN, gtwN, M + 1);
+  __syncthreads();
+  const RowLds L{sm.tile + q * M, q & 15};
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
-  unpack_store<M, TP, NT>(lds, S, twN, g, z, y0);
+  unpack_store<M, TP, NT>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-// C2R of a spectrum tile -> v[e] = (x[2m], x[2m+1]), m = t + T*e
 template <int M, int E, int TP>
-__device__ __forceinline__ void inverse_rows(float2 (&v)[E], float2 *lds, const RowLds &L,
-                                             int t, const float2 *__restrict__ S,
-                                             const float2 *__restrict__ twM,
-                                             const float2 *__restrict__ twN, const Geom &g,
-                                             int z, int y0) {
-  constexpr int NT = TP * (M / E);
-  load_spectrum<M, TP, NT>(lds, S, g, z, y0);
-  __syncthreads();
-  pack_inverse<M, E>(v, L, t, twN);
-  __syncthreads();
-  fft_stages<M, E, 1, +1>(v, t, L, twM, typename XCfg<M>::Rad{});
-}
-
-template <int M, int TP, int E = XCfg<M>::E>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
-            const float *__restrict__ cptr, const float2 *__restrict__ twM,
-            const float2 *__restrict__ twN, Geom g) {
+            const float *__restrict__ cptr, const float2 *__restrict__ gtwM,
+            const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
-  __shared__ float2 lds[TP * M];
+  __shared__ XSmem<M, TP> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
   const int z = blockIdx.x / ntile;
   const int y0 = (blockIdx.x % ntile) * TP;
   const int y = y0 + q;
   const bool valid = y < g.iy;
-  const RowLds L{lds + q * M, q & 15};
+  // the observation row is needed only after the inverse transform; its
+  // loads are issued first so they overlap the spectrum load and the FFT
+  float2 o[E];
+  load_row_pairs<E, T>(o, obs + ((long long)z * g.iy + y) * g.ix, valid, t, g.ix);
+  stage_table<NT>(sm.twM, gtwM, M);
+  stage_table<NT>(sm.twN, gtwN, M + 1);
+  load_spectrum<M, TP, NT>(sm.tile, S, g, z, y0);
+  __syncthreads();
+  const RowLds L{sm.tile + q * M, q & 15};
   float2 v[E];
-  inverse_rows<M, E, TP>(v, lds, L, t, S, twM, twN, g, z, y0);
+  pack_inverse<M, E>(v, L, t, sm.twN);
+  __syncthreads();
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
   // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
   // |s1| < 1e-5 (compared in double) gives 0, as the reference's ML step
   // (src/ComputePrimitives.cxx:143-148, src/MaximumLikelihoodDeconvolve.cxx:62)
   const float c = cptr[0];
-  const float *orow = obs + ((long long)z * g.iy + y) * g.ix;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     float r0 = 0.f, r1 = 0.f;
     if (valid && n0 < g.ix) {
       const float s = c * v[e].x;
-      r0 = ((double)fabsf(s) < 1e-5) ? 0.f : orow[n0] / s;
+      r0 = ((double)fabsf(s) < 1e-5) ? 0.f : o[e].x / s;
     }
     if (valid && n0 + 1 < g.ix) {
       const float s = c * v[e].y;
-      r1 = ((double)fabsf(s) < 1e-5) ? 0.f : orow[n0 + 1] / s;
+      r1 = ((double)fabsf(s) < 1e-5) ? 0.f : o[e].y / s;
     }
     v[e] = make_float2(r0, r1);
   }
-  fft_stages<M, E, 1, -1>(v, t, L, twM, typename XCfg<M>::Rad{});
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
-  unpack_store<M, TP, NT>(lds, S, twN, g, z, y0);
+  unpack_store<M, TP, NT>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-template <int M, int TP, int E = XCfg<M>::E>
+template <int M, int E, int TP>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
-             const float2 *__restrict__ twM, const float2 *__restrict__ twN, Geom g) {
+             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
-  __shared__ float2 lds[TP * M];
+  __shared__ XSmem<M, TP> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
   const int z = blockIdx.x / ntile;
   const int y0 = (blockIdx.x % ntile) * TP;
   const int y = y0 + q;
   const bool valid = y < g.iy;
-  const RowLds L{lds + q * M, q & 15};
+  float *urow = u + ((long long)z * g.iy + y) * g.ix;
+  float2 uo[E];
+  load_row_pairs<E, T>(uo, urow, valid, t, g.ix);
+  stage_table<NT>(sm.twM, gtwM, M);
+  stage_table<NT>(sm.twN, gtwN, M + 1);
+  load_spectrum<M, TP, NT>(sm.tile, S, g, z, y0);
+  __syncthreads();
+  const RowLds L{sm.tile + q * M, q & 15};
   float2 v[E];
-  inverse_rows<M, E, TP>(v, lds, L, t, S, twM, twN, g, z, y0);
+  pack_inverse<M, E>(v, L, t, sm.twN);
+  __syncthreads();
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
   // u' = (c * u) * s2: the reference's g' = g * s2 with g = c * u stored
   // rounded (src/MaximumLikelihoodDeconvolve.cxx:69-78)
   const float c = cptr[0];
-  float *urow = u + ((long long)z * g.iy + y) * g.ix;
   double acc = 0.0;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     if (valid && n0 < g.ix) {
-      const float gk = c * urow[n0];
-      const float un = gk * v[e].x;
+      const float un = (c * uo[e].x) * v[e].x;
       urow[n0] = un;
       acc += (double)un;
     }
     if (valid && n0 + 1 < g.ix) {
-      const float gk = c * urow[n0 + 1];
-      const float un = gk * v[e].y;
+      const float un = (c * uo[e].y) * v[e].y;
       urow[n0 + 1] = un;
       acc += (double)un;
     }
   }
-  double *sh = reinterpret_cast<double *>(lds);
+  double *sh = reinterpret_cast<double *>(sm.tile);
   __syncthreads();
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -496,12 +499,14 @@ __global__ void __launch_bounds__(TP *(M / E))
 
 // ------------------------------------------------------------- y and z
 
-template <int N, int SIGN, int E = YCfg<N>::E, int PPW = YCfg<N>::PPW>
+template <int N, int E, int PPW, int SIGN>
 __global__ void __launch_bounds__(PPW *(N / E))
-    k_y(float2 *__restrict__ S, const float2 *__restrict__ tw, long long npencils,
+    k_y(float2 *__restrict__ S, const float2 *__restrict__ gtw, long long npencils,
         int Lin, int Lout) {
   constexpr int T = N / E;
+  constexpr int NT = PPW * T;
   __shared__ float2 lds[PPW * N];
+  __shared__ float2 tw[N];
   const int pl = threadIdx.x / T, t = threadIdx.x % T;
   const long long p = (long long)blockIdx.x * PPW + pl;
   const bool valid = p < npencils;
@@ -512,8 +517,10 @@ __global__ void __launch_bounds__(PPW *(N / E))
     const int j = t + T * e;
     v[e] = (valid && j < Lin) ? base[j] : make_float2(0.f, 0.f);
   }
+  stage_table<NT>(tw, gtw, N);
+  __syncthreads();
   const PencilLds L{lds + pl * N};
-  fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename YCfg<N>::Rad{});
+  fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename Plan<N, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int j = t + T * e;
@@ -521,12 +528,16 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
 }
 
-template <int N, int TP, int E = ZCfg<N>::E>
+// PRE: the OTF column is loaded into registers at the start, overlapping the
+// forward transform (more registers, more loads in flight).
+template <int N, int E, int TP, bool PRE>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
-        const float2 *__restrict__ tw, Geom g, float scale) {
+        const float2 *__restrict__ gtw, Geom g, float scale) {
   constexpr int T = N / E;
+  constexpr int NT = TP * T;
   __shared__ float2 lds[TP * N];
+  __shared__ float2 tw[N];
   const int q = threadIdx.x % TP, t = threadIdx.x / TP;
   const int ntile = g.ny / TP;
   const int kx = blockIdx.x / ntile;
@@ -538,17 +549,27 @@ __global__ void __launch_bounds__(TP *(N / E))
     const int z = t + T * e;
     v[e] = (z < g.iz) ? S[z * g.sz + col] : make_float2(0.f, 0.f);
   }
+  float2 h[PRE ? E : 1];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int e = 0; e < E; e++) h[e] = otf[(t + T * e) * g.sz + col];
+  }
+  stage_table<NT>(tw, gtw, N);
+  __syncthreads();
   const InterLds<TP> L{lds + q};
-  fft_stages<N, E, 1, -1>(v, t, L, tw, typename ZCfg<N>::Rad{});
+  fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
   // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    const int kz = t + T * e;
-    const float2 h = otf[kz * g.sz + col];
+    float2 hv;
+    if constexpr (PRE)
+      hv = h[e];
+    else
+      hv = otf[(t + T * e) * g.sz + col];
     const float2 a = v[e];
-    v[e] = make_float2(scale * (a.x * h.x - a.y * h.y), scale * (a.x * h.y + a.y * h.x));
+    v[e] = make_float2(scale * (a.x * hv.x - a.y * hv.y), scale * (a.x * hv.y + a.y * hv.x));
   }
-  fft_stages<N, E, 1, +1>(v, t, L, tw, typename ZCfg<N>::Rad{});
+  fft_stages<N, E, 1, +1>(v, t, L, tw, typename Plan<N, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int z = t + T * e;
@@ -565,8 +586,11 @@ const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod", "y_inv",
 constexpr int kTPx = 16;  // rows per x tile: 16 consecutive y -> 128 B
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
-struct Launch {
-  unsigned grid = 0, block = 0;
+// Launch shape of the fused kernels; defaults per size, overridable with
+// CLARITY_FAST_TUNE="xe=16,yp=8,ze=8,zp=1" (x elements per thread, y pencils
+// per workgroup, z elements per thread, z OTF prefetch).
+struct Tune {
+  int xe = 0, yp = 0, ze = 0, zp = -1;
 };
 
 bool x_supported(int M) {
@@ -587,76 +611,142 @@ struct Bufs {
   const float2 *twM, *twN, *twY, *twZ;
 };
 
-template <int M>
-void launch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  constexpr int NT = kTPx * (M / XCfg<M>::E);
+template <int M, int E>
+void launch_x_e(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  constexpr int NT = kTPx * (M / E);
   const unsigned grid = (unsigned)g.iz * ((g.iy + kTPx - 1) / kTPx);
   if (kind == KX_FWD)
-    hipLaunchKernelGGL((k_x_fwd<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+    hipLaunchKernelGGL((k_x_fwd<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
   else if (kind == KX_MID)
-    hipLaunchKernelGGL((k_x_mid<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+    hipLaunchKernelGGL((k_x_mid<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
   else
-    hipLaunchKernelGGL((k_x_last<M, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
 }
 
-void dispatch_x(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  switch (g.nx / 2) {
-    case 64: launch_x<64>(kind, g, b, s); break;
-    case 96: launch_x<96>(kind, g, b, s); break;
-    case 128: launch_x<128>(kind, g, b, s); break;
-    case 192: launch_x<192>(kind, g, b, s); break;
-    case 384: launch_x<384>(kind, g, b, s); break;
-    case 256: launch_x<256>(kind, g, b, s); break;
-    case 512: launch_x<512>(kind, g, b, s); break;
+template <int M>
+void launch_x(int e, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  if constexpr (M % 3 == 0) {
+    launch_x_e<M, 12>(kind, g, b, s);
+  } else if constexpr (M == 64) {
+    launch_x_e<M, 8>(kind, g, b, s);
+  } else {
+    if (e == 16)
+      launch_x_e<M, 16>(kind, g, b, s);
+    else
+      launch_x_e<M, 8>(kind, g, b, s);
   }
 }
 
-template <int N>
-void launch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
-  constexpr int E = YCfg<N>::E, PPW = YCfg<N>::PPW;
+void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  switch (g.nx / 2) {
+    case 64: launch_x<64>(tu.xe, kind, g, b, s); break;
+    case 96: launch_x<96>(tu.xe, kind, g, b, s); break;
+    case 128: launch_x<128>(tu.xe, kind, g, b, s); break;
+    case 192: launch_x<192>(tu.xe, kind, g, b, s); break;
+    case 256: launch_x<256>(tu.xe, kind, g, b, s); break;
+    case 384: launch_x<384>(tu.xe, kind, g, b, s); break;
+    case 512: launch_x<512>(tu.xe, kind, g, b, s); break;
+  }
+}
+
+template <int N, int E, int PPW>
+void launch_y_p(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   const long long np = (long long)g.iz * g.hx;
   const unsigned grid = (unsigned)((np + PPW - 1) / PPW);
   if (fwd)
-    hipLaunchKernelGGL((k_y<N, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+    hipLaunchKernelGGL((k_y<N, E, PPW, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
                        b.twY, np, g.iy, N);
   else
-    hipLaunchKernelGGL((k_y<N, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
+    hipLaunchKernelGGL((k_y<N, E, PPW, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
                        b.twY, np, N, g.iy);
 }
 
-void dispatch_y(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
+template <int N>
+void launch_y(int ppw, bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
+  constexpr int E = (N % 3 == 0) ? 12 : (N == 128 || N == 512) ? 8 : 16;
+  constexpr int T = N / E;
+  // pencils per workgroup: 2, 4 or 8 of them, at most 1024 threads
+  if (ppw >= 8 && 8 * T <= 1024)
+    launch_y_p<N, E, 8>(fwd, g, b, s);
+  else if (ppw >= 4 || 2 * T < 64)
+    launch_y_p<N, E, (4 * T <= 1024 ? 4 : 2)>(fwd, g, b, s);
+  else
+    launch_y_p<N, E, 2>(fwd, g, b, s);
+}
+
+void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   switch (g.ny) {
-    case 128: launch_y<128>(fwd, g, b, s); break;
-    case 192: launch_y<192>(fwd, g, b, s); break;
-    case 384: launch_y<384>(fwd, g, b, s); break;
-    case 768: launch_y<768>(fwd, g, b, s); break;
-    case 256: launch_y<256>(fwd, g, b, s); break;
-    case 512: launch_y<512>(fwd, g, b, s); break;
-    case 1024: launch_y<1024>(fwd, g, b, s); break;
-    case 2048: launch_y<2048>(fwd, g, b, s); break;
+    case 128: launch_y<128>(tu.yp, fwd, g, b, s); break;
+    case 192: launch_y<192>(tu.yp, fwd, g, b, s); break;
+    case 256: launch_y<256>(tu.yp, fwd, g, b, s); break;
+    case 384: launch_y<384>(tu.yp, fwd, g, b, s); break;
+    case 512: launch_y<512>(tu.yp, fwd, g, b, s); break;
+    case 768: launch_y<768>(tu.yp, fwd, g, b, s); break;
+    case 1024: launch_y<1024>(tu.yp, fwd, g, b, s); break;
+    case 2048: launch_y<2048>(tu.yp, fwd, g, b, s); break;
   }
+}
+
+template <int N, int E, bool PRE>
+void launch_z_e(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  const unsigned grid = (unsigned)(g.hx * (g.ny / kTPz));
+  hipLaunchKernelGGL((k_z<N, E, kTPz, PRE>), dim3(grid), dim3(kTPz * (N / E)), 0, s,
+                     b.S, b.otf, b.twZ, g, scale);
 }
 
 template <int N>
-void launch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
-  constexpr int E = ZCfg<N>::E;
-  const unsigned grid = (unsigned)(g.hx * (g.ny / kTPz));
-  hipLaunchKernelGGL((k_z<N, kTPz>), dim3(grid), dim3(kTPz * (N / E)), 0, s, b.S,
-                     b.otf, b.twZ, g, scale);
+void launch_z(int e, int pre, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  if constexpr (N % 3 == 0) {
+    if (pre) launch_z_e<N, 12, true>(g, b, scale, s);
+    else launch_z_e<N, 12, false>(g, b, scale, s);
+  } else if constexpr (N == 64) {
+    if (pre) launch_z_e<N, 8, true>(g, b, scale, s);
+    else launch_z_e<N, 8, false>(g, b, scale, s);
+  } else {
+    if (e == 8) {
+      if (pre) launch_z_e<N, 8, true>(g, b, scale, s);
+      else launch_z_e<N, 8, false>(g, b, scale, s);
+    } else {
+      if (pre) launch_z_e<N, 16, true>(g, b, scale, s);
+      else launch_z_e<N, 16, false>(g, b, scale, s);
+    }
+  }
 }
 
-void dispatch_z(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   switch (g.nz) {
-    case 64: launch_z<64>(g, b, scale, s); break;
-    case 96: launch_z<96>(g, b, scale, s); break;
-    case 128: launch_z<128>(g, b, scale, s); break;
-    case 192: launch_z<192>(g, b, scale, s); break;
-    case 256: launch_z<256>(g, b, scale, s); break;
-    case 512: launch_z<512>(g, b, scale, s); break;
+    case 64: launch_z<64>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 96: launch_z<96>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 128: launch_z<128>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 192: launch_z<192>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 256: launch_z<256>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 512: launch_z<512>(tu.ze, tu.zp, g, b, scale, s); break;
   }
+}
+
+// Default shapes, then the environment override.
+Tune pick_tune(const Geom &g) {
+  Tune t;
+  t.xe = (g.nx / 2 == 256) ? 16 : 8;
+  t.yp = 4;
+  t.ze = (g.nz == 128) ? 8 : 16;
+  t.zp = 0;
+  if (const char *s = std::getenv("CLARITY_FAST_TUNE")) {
+    int v;
+    const char *p = s;
+    while (*p) {
+      if (std::sscanf(p, "xe=%d", &v) == 1) t.xe = v;
+      else if (std::sscanf(p, "yp=%d", &v) == 1) t.yp = v;
+      else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
+      else if (std::sscanf(p, "zp=%d", &v) == 1) t.zp = v;
+      while (*p && *p != ',') p++;
+      if (*p == ',') p++;
+    }
+  }
+  return t;
 }
 
 // --------------------------------------------------------------- engine
@@ -670,6 +760,7 @@ class FastRL : public RLEngine {
     g_.hx = g_.nx / 2 + 1;
     g_.sz = (long long)g_.hx * g_.ny;
     nparts_ = g_.iz * ((g_.iy + kTPx - 1) / kTPx);
+    tu_ = pick_tune(g_);
   }
   ~FastRL() override {
     for (auto &ev : ev_pool_) {
@@ -842,10 +933,10 @@ class FastRL : public RLEngine {
     switch (kind) {
       case KX_FWD:
       case KX_MID:
-      case KX_LAST: dispatch_x(kind, g_, b_, s); break;
-      case KY_FWD: dispatch_y(true, g_, b_, s); break;
-      case KY_INV: dispatch_y(false, g_, b_, s); break;
-      case KZ_MOD: dispatch_z(g_, b_, scale, s); break;
+      case KX_LAST: dispatch_x(tu_, kind, g_, b_, s); break;
+      case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
+      case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
+      case KZ_MOD: dispatch_z(tu_, g_, b_, scale, s); break;
       case KFINISH:
         CL_TRY(k_finish_sum(b_.parts, nparts_, b_.scal + 1, b_.scal, b_.scal + 2, s));
         break;
@@ -860,6 +951,7 @@ class FastRL : public RLEngine {
   }
 
   Geom g_{};
+  Tune tu_{};
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, otf_, parts_, scal_;

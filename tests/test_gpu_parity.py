@@ -258,9 +258,11 @@ def test_blind_rl_vs_oracle(clarity_gpu, cfg1):
     """Alternating blind RL amplifies float32 rounding: on this case the
     oracle run with a float32 FFT (the FFTW3 API, as the reference computes)
     differs from the float64-FFT oracle by 1.05e-3 (image) and 6.9e-3
-    (kernel) after 2 x 2 iterations.  The bound is the larger of 1e-4 and
-    twice that float32-FFT spread, measured here when the FFTW3 API is
-    present (else the values above)."""
+    (kernel) after 2 x 2 iterations with MKL on one host and by 2.7e-4 and
+    3.2e-3 on another: two float32 FFTs of the same library already differ
+    by 2x.  The bound is the larger of 1e-4 and four times that float32-FFT
+    spread, measured here when the FFTW3 API is present (else the values
+    above)."""
     api = clarity_gpu
     k0 = synth.blind_initial_kernel(1, cfg1.psf.shape)
     o, h = api.BlindRL(cfg1.obs, k0, 2, 2)
@@ -273,8 +275,8 @@ def test_blind_rl_vs_oracle(clarity_gpu, cfg1):
     eo, eh = rel_l2(o, ro), rel_l2(h, rh)
     print(f"[blind] gpu err o {eo:.2e} h {eh:.2e}; f32-FFT spread o {spread_o:.2e} "
           f"h {spread_h:.2e}")
-    assert eo <= max(1e-4, 2 * spread_o)
-    assert eh <= max(1e-4, 2 * spread_h)
+    assert eo <= max(1e-4, 4 * spread_o)
+    assert eh <= max(1e-4, 4 * spread_h)
     assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
     assert np.all(h >= 0)
 
