@@ -358,6 +358,24 @@ __device__ __forceinline__ void load_row_pairs(float2 (&v)[E], const float *row,
 
 // x kernels share one shared-memory block: the TP-row tile, then the M-point
 // and (M+1)-entry twiddle tables.
+// Tile index of this workgroup.  With 16-row tiles a tile spans 128 bytes
+// of every spectrum row and blocks map to tiles in order.  With 8-row tiles
+// two tiles share each 128-byte span; blocks b and b + 8, which the
+// dispatcher sends to the same XCD (round-robin over 8) at nearly the same
+// time, take the two halves, so the line is fetched into that XCD's L2 once.
+// The grid is padded to a multiple of 16; padding blocks get -1.
+template <int TP>
+__device__ __forceinline__ int x_tile(int ntiles) {
+  const int b = blockIdx.x;
+  if constexpr (TP == 16) {
+    return b < ntiles ? b : -1;
+  } else {
+    const int loc = b & 15;
+    const int t = (b - loc) + (loc < 8 ? 2 * loc : 2 * (loc - 8) + 1);
+    return t < ntiles ? t : -1;
+  }
+}
+
 template <int M, int TP>
 struct XSmem {
   float2 tile[TP * M];
@@ -374,8 +392,10 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int z = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
+  const int tile = x_tile<TP>(g.iz * ntile);
+  if (tile < 0) return;  // whole workgroup: padding block of the remap
+  const int z = tile / ntile;
+  const int y0 = (tile % ntile) * TP;
   const int y = y0 + q;
   float2 v[E];
   load_row_pairs<E, T>(v, u + ((long long)z * g.iy + y) * g.ix, y < g.iy, t, g.ix);
@@ -400,8 +420,10 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int z = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
+  const int tile = x_tile<TP>(g.iz * ntile);
+  if (tile < 0) return;  // whole workgroup: padding block of the remap
+  const int z = tile / ntile;
+  const int y0 = (tile % ntile) * TP;
   const int y = y0 + q;
   const bool valid = y < g.iy;
   // the observation row is needed only after the inverse transform; its
@@ -452,8 +474,10 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int z = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
+  const int tile = x_tile<TP>(g.iz * ntile);
+  if (tile < 0) return;  // whole workgroup: padding block of the remap
+  const int z = tile / ntile;
+  const int y0 = (tile % ntile) * TP;
   const int y = y0 + q;
   const bool valid = y < g.iy;
   float *urow = u + ((long long)z * g.iy + y) * g.ix;
@@ -494,7 +518,7 @@ __global__ void __launch_bounds__(TP *(M / E))
     if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) partials[tile] = sh[0];
 }
 
 // ------------------------------------------------------------- y and z
@@ -583,14 +607,14 @@ enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KNUM };
 const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod", "y_inv",
                                   "x_mid", "x_last", "finish"};
 
-constexpr int kTPx = 16;  // rows per x tile: 16 consecutive y -> 128 B
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
 // Launch shape of the fused kernels; defaults per size, overridable with
-// CLARITY_FAST_TUNE="xe=16,yp=8,ze=8,zp=1" (x elements per thread, y pencils
-// per workgroup, z elements per thread, z OTF prefetch).
+// CLARITY_FAST_TUNE="xe=16,xl=16,xt=8,yp=8,ze=8,zp=1" (x elements per thread
+// for x_fwd/x_mid and for x_last, x tile rows, y pencils per workgroup, z
+// elements per thread, z OTF prefetch).
 struct Tune {
-  int xe = 0, yp = 0, ze = 0, zp = -1;
+  int xe = 0, xle = 0, xt = 16, yp = 0, ze = 0, zp = -1;
 };
 
 bool x_supported(int M) {
@@ -611,44 +635,57 @@ struct Bufs {
   const float2 *twM, *twN, *twY, *twZ;
 };
 
-template <int M, int E>
-void launch_x_e(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  constexpr int NT = kTPx * (M / E);
-  const unsigned grid = (unsigned)g.iz * ((g.iy + kTPx - 1) / kTPx);
+// Number of x tiles (= x_last partial sums) for a tile height.
+int x_tiles(const Geom &g, int tp) { return g.iz * ((g.iy + tp - 1) / tp); }
+
+template <int M, int E, int TP>
+void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  constexpr int NT = TP * (M / E);
+  const int nt = x_tiles(g, TP);
+  const unsigned grid = (unsigned)(TP == 16 ? nt : (nt + 15) / 16 * 16);
   if (kind == KX_FWD)
-    hipLaunchKernelGGL((k_x_fwd<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+    hipLaunchKernelGGL((k_x_fwd<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
   else if (kind == KX_MID)
-    hipLaunchKernelGGL((k_x_mid<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+    hipLaunchKernelGGL((k_x_mid<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
   else
-    hipLaunchKernelGGL((k_x_last<M, E, kTPx>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
 }
 
+template <int M, int E>
+void launch_x_e(int tp, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  if (tp == 8)
+    launch_x_t<M, E, 8>(kind, g, b, s);
+  else
+    launch_x_t<M, E, 16>(kind, g, b, s);
+}
+
 template <int M>
-void launch_x(int e, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_x(int e, int tp, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   if constexpr (M % 3 == 0) {
-    launch_x_e<M, 12>(kind, g, b, s);
+    launch_x_e<M, 12>(tp, kind, g, b, s);
   } else if constexpr (M == 64) {
-    launch_x_e<M, 8>(kind, g, b, s);
+    launch_x_e<M, 8>(tp, kind, g, b, s);
   } else {
     if (e == 16)
-      launch_x_e<M, 16>(kind, g, b, s);
+      launch_x_e<M, 16>(tp, kind, g, b, s);
     else
-      launch_x_e<M, 8>(kind, g, b, s);
+      launch_x_e<M, 8>(tp, kind, g, b, s);
   }
 }
 
 void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  const int e = (kind == KX_LAST) ? tu.xle : tu.xe;
   switch (g.nx / 2) {
-    case 64: launch_x<64>(tu.xe, kind, g, b, s); break;
-    case 96: launch_x<96>(tu.xe, kind, g, b, s); break;
-    case 128: launch_x<128>(tu.xe, kind, g, b, s); break;
-    case 192: launch_x<192>(tu.xe, kind, g, b, s); break;
-    case 256: launch_x<256>(tu.xe, kind, g, b, s); break;
-    case 384: launch_x<384>(tu.xe, kind, g, b, s); break;
-    case 512: launch_x<512>(tu.xe, kind, g, b, s); break;
+    case 64: launch_x<64>(e, tu.xt, kind, g, b, s); break;
+    case 96: launch_x<96>(e, tu.xt, kind, g, b, s); break;
+    case 128: launch_x<128>(e, tu.xt, kind, g, b, s); break;
+    case 192: launch_x<192>(e, tu.xt, kind, g, b, s); break;
+    case 256: launch_x<256>(e, tu.xt, kind, g, b, s); break;
+    case 384: launch_x<384>(e, tu.xt, kind, g, b, s); break;
+    case 512: launch_x<512>(e, tu.xt, kind, g, b, s); break;
   }
 }
 
@@ -731,6 +768,8 @@ void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipSt
 Tune pick_tune(const Geom &g) {
   Tune t;
   t.xe = (g.nx / 2 == 256) ? 16 : 8;
+  t.xle = t.xe;
+  t.xt = 16;
   t.yp = 4;
   t.ze = (g.nz == 128) ? 8 : 16;
   t.zp = 0;
@@ -739,6 +778,8 @@ Tune pick_tune(const Geom &g) {
     const char *p = s;
     while (*p) {
       if (std::sscanf(p, "xe=%d", &v) == 1) t.xe = v;
+      else if (std::sscanf(p, "xl=%d", &v) == 1) t.xle = v;
+      else if (std::sscanf(p, "xt=%d", &v) == 1) t.xt = v;
       else if (std::sscanf(p, "yp=%d", &v) == 1) t.yp = v;
       else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
       else if (std::sscanf(p, "zp=%d", &v) == 1) t.zp = v;
@@ -759,10 +800,11 @@ class FastRL : public RLEngine {
     g_.nx = geom.wd.x; g_.ny = geom.wd.y; g_.nz = geom.wd.z;
     g_.hx = g_.nx / 2 + 1;
     g_.sz = (long long)g_.hx * g_.ny;
-    nparts_ = g_.iz * ((g_.iy + kTPx - 1) / kTPx);
     tu_ = pick_tune(g_);
+    nparts_ = x_tiles(g_, tu_.xt == 8 ? 8 : 16);
   }
   ~FastRL() override {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     for (auto &ev : ev_pool_) {
       (void)hipEventDestroy(ev.first);
       (void)hipEventDestroy(ev.second);
@@ -837,18 +879,46 @@ class FastRL : public RLEngine {
     return CLARITY_SUCCESS;
   }
 
+  // One RL iteration = ten launches.  Without per-kernel timing the
+  // sequence is captured once into a HIP graph and replayed, which removes
+  // the host launch cost (it matters for small volumes).  The graph holds
+  // kernel nodes only; every buffer it touches lives as long as the engine.
   ClarityResult_t iterate(int n, hipStream_t s) override {
-    for (int it = 0; it < n; it++) {
-      CL_TRY(run(KX_FWD, s));
-      for (int half = 0; half < 2; half++) {
-        CL_TRY(run(KY_FWD, s));
-        CL_TRY(run(KZ_MOD, s));
-        CL_TRY(run(KY_INV, s));
-        CL_TRY(run(half == 0 ? KX_MID : KX_LAST, s));
-      }
-      CL_TRY(run(KFINISH, s));
+    if (n <= 0) return CLARITY_SUCCESS;
+    if (timing_ || !use_graph_) {
+      for (int it = 0; it < n; it++) CL_TRY(enqueue_iteration(s));
+      return CLARITY_SUCCESS;
     }
+    if (!graph_exec_) {
+      hipGraph_t graph = nullptr;
+      CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      ClarityResult_t r = enqueue_iteration(s);
+      hipError_t e = hipStreamEndCapture(s, &graph);
+      if (r != CLARITY_SUCCESS) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r;
+      }
+      CL_HIP(e);
+      e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        graph_exec_ = nullptr;
+        return from_hip(e);
+      }
+    }
+    for (int it = 0; it < n; it++) CL_HIP(hipGraphLaunch(graph_exec_, s));
     return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t enqueue_iteration(hipStream_t s) {
+    CL_TRY(run(KX_FWD, s));
+    for (int half = 0; half < 2; half++) {
+      CL_TRY(run(KY_FWD, s));
+      CL_TRY(run(KZ_MOD, s));
+      CL_TRY(run(KY_INV, s));
+      CL_TRY(run(half == 0 ? KX_MID : KX_LAST, s));
+    }
+    return run(KFINISH, s);
   }
 
   ClarityResult_t estimate(float *out_d, hipStream_t s) override {
@@ -952,6 +1022,8 @@ class FastRL : public RLEngine {
 
   Geom g_{};
   Tune tu_{};
+  bool use_graph_ = std::getenv("CLARITY_NO_GRAPH") == nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, otf_, parts_, scal_;
