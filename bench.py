@@ -111,6 +111,100 @@ def cpu_baseline(obs, psf, iterations, threads):
     return loop, total, backend
 
 
+def dense_bytes(shape, passes_real, passes_complex):
+    """Bytes of `passes_real` real and `passes_complex` half-spectrum sweeps of
+    the work domain of an image `shape` with a same-size kernel."""
+    nz, ny, nx = (2 * s for s in shape)
+    return 4.0 * nx * ny * nz * passes_real + 8.0 * (nx // 2 + 1) * ny * nz * passes_complex
+
+
+def suite(api, threads):
+    """Every other driver on the path, GPU vs the CPU oracle on the same
+    inputs (config-2 volume 256x256x64 with a same-size kernel; batched RL
+    on config-4's 48 uneven volumes).  GPU times are whole C-API calls with
+    host buffers (upload, pad, compute, clip, download); the dense-model
+    bandwidth counts only the work-domain sweeps of the algorithm."""
+    import oracle
+    from clarity_amd import synth
+    oracle.set_fft_backend(oracle.FFT_FFTW_API)
+    oracle.set_reduce_mode(oracle.REDUCE_SERIAL_F32)
+    oracle.set_threads(threads)
+    backend = oracle.fft_backend_name()
+    shape = (64, 256, 256)
+    rng = np.random.default_rng(2)
+    obj = synth.beads_and_filaments(shape, 2000, 300, rng)
+    psf = synth.gaussian_kernel(shape, 2.0, 4.5, side_lobe=0.1)
+    obs = synth.poisson(api.Convolve(obj, psf), rng)
+
+    def gpu_time(fn, reps=3):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    def cpu_time(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    def emit(name, g, c, byts, note=""):
+        line = {"suite": name, "gpu_ms": round(g * 1e3, 3), "cpu_ms": round(c * 1e3, 1),
+                "speedup": round(c / g, 1), "cpu_kind": "port", "cpu_fft": backend,
+                "cores": threads}
+        if byts:
+            line["dense_GBps"] = round(byts / g / 1e9, 1)
+        if note:
+            line["note"] = note
+        print(json.dumps(line), flush=True)
+
+    emit("convolve 256x256x64", gpu_time(lambda: api.Convolve(obj, psf)),
+         cpu_time(lambda: oracle.convolve(obj, psf)), dense_bytes(shape, 3, 14))
+    emit("wiener eps=1e-3 256x256x64", gpu_time(lambda: api.WienerDeconvolve(obs, psf, 1e-3)),
+         cpu_time(lambda: oracle.wiener(obs, psf, 1e-3)), dense_bytes(shape, 3, 14))
+    g = gpu_time(lambda: api.JansenVanCittertDeconvolve(obs, psf, 50), reps=1)
+    c1 = cpu_time(lambda: oracle.jvc(obs, psf, 1))
+    c3 = cpu_time(lambda: oracle.jvc(obs, psf, 3))
+    emit("jvc 50 iterations 256x256x64", g, c1 + 49 * max(c3 - c1, 0.0) / 2,
+         dense_bytes(shape, 4 * 50, 9 * 50),
+         "cpu: 1-iteration call + 49 x per-iteration cost from a 3-iteration call")
+    g = gpu_time(lambda: api.MaximumLikelihoodDeconvolve(obs, psf, 100), reps=1)
+    oracle.ml(obs, psf, 2)
+    loop = oracle.last_loop_seconds() / 2
+    emit("rl 100 iterations 256x256x64 (C API, host buffers)", g, 100 * loop,
+         dense_bytes(shape, 4 * 100, 18 * 100), "cpu: 100 x per-iteration loop time of 2")
+    k0 = synth.gaussian_kernel(shape, 2.4, 5.4, side_lobe=0.1)
+    g = gpu_time(lambda: api.BlindRL(obs, k0, 4, 5), reps=1)
+    cb = cpu_time(lambda: oracle.blind_rl(obs, k0, 1, 1))
+    emit("blind rl 4 outer x 5 inner 256x256x64", g, 20 * cb, 0,
+         "cpu: 20 x the cost of a 1 x 1 call")
+    # config 4: 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
+    rng = np.random.default_rng(4)
+    imgs, ks = [], []
+    for _ in range(48):
+        s = (int(rng.choice([32, 48, 64, 96])), int(rng.choice([64, 96, 128, 192, 256])),
+             int(rng.choice([64, 96, 128, 192, 256])))
+        f = np.prod(s) / np.prod(shape)
+        o = synth.beads_and_filaments(s, int(2000 * f) + 1, int(300 * f) + 1, rng)
+        k = synth.gaussian_kernel(s, rng.uniform(1.2, 2.5), rng.uniform(2.5, 5.0))
+        imgs.append(synth.poisson(api.Convolve(o, k), rng))
+        ks.append(k)
+    vox = sum(int(np.prod(a.shape)) for a in imgs)
+    g = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 50), reps=1)
+    t_cpu = 0.0
+    for a, k in zip(imgs[:3], ks[:3]):
+        oracle.ml(a, k, 1)
+        t_cpu += oracle.last_loop_seconds() * vox / sum(int(np.prod(x.shape)) for x in imgs[:3])
+    line = {"suite": "batched rl 48 uneven volumes x 50 iterations",
+            "gpu_ms": round(g * 1e3, 2), "Mvoxel_iters_per_s": round(vox * 50 / g / 1e6, 1),
+            "cpu_Mvoxel_iters_per_s": round(vox / t_cpu / 1e6, 2), "voxels": vox,
+            "cpu_note": "1 iteration on 3 of the volumes, scaled by voxels",
+            "cpu_fft": backend, "cores": threads}
+    print(json.dumps(line), flush=True)
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +218,17 @@ def main():
                     help="after the run, copy this many MiB device-to-device 3 times "
                          "(a plain copy kernel of known size, to calibrate rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--suite", action="store_true",
+                    help="instead of the headline line, time every other driver on "
+                         "the path (GPU vs CPU oracle), one JSON line each")
     args = ap.parse_args()
+
+    if args.suite:
+        from clarity_amd import api
+        api.Register()
+        suite(api, min(16, os.cpu_count() or 1))
+        api.UnRegister()
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this "

@@ -157,22 +157,43 @@ __device__ __forceinline__ float2 tw_load(const float2 *__restrict__ tw, int idx
 
 // ------------------------------------------------------------- LDS layouts
 
+// LDS exchanges between the lanes of one wavefront: the lanes run in
+// lockstep and a wavefront's LDS operations complete in order, so a fence
+// that keeps the compiler from reordering them is all that is needed.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row q of an x-pass tile, XOR-swizzled inside 16-element blocks so that a
-// column access (same pos, 16 consecutive rows) hits distinct banks.
+// column access (same pos, 16 consecutive rows) hits distinct banks.  A row
+// is transformed by T <= 64 consecutive lanes, i.e. inside one wavefront.
 struct RowLds {
   float2 *base;
   int sw;
   __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos ^ sw]; }
+  __device__ __forceinline__ void sync() const { wave_sync(); }
 };
-// Pencil q of an interleaved tile: element pos at pos*TP + q.
+// Pencil q of an interleaved tile: element pos at pos*TP + q.  A pencil's
+// threads are spread over the workgroup.
 template <int TP>
 struct InterLds {
   float2 *base;
   __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos * TP]; }
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
+// A contiguous pencil transformed by T consecutive lanes; WAVE when T <= 64.
+template <bool WAVE>
 struct PencilLds {
   float2 *base;
   __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos]; }
+  __device__ __forceinline__ void sync() const {
+    if constexpr (WAVE)
+      wave_sync();
+    else
+      __syncthreads();
+  }
 };
 
 // ------------------------------------------------------------ FFT engine
@@ -198,8 +219,10 @@ using DefaultRadices = typename RadixList<N, E, 1, false>::type;
 
 // Stockham stages from span P on, one per radix of the list.  Thread t of a
 // pencil holds x[t + T*e] in v[e] on entry; on exit v[e] holds X[t + T*e]
-// (natural order).  All threads of the workgroup must call this together
-// (it synchronises).
+// (natural order).  Stages exchange through the accessor's LDS and
+// synchronise with its sync(): a wavefront-local barrier when every pencil
+// lies inside one wavefront, else a workgroup barrier (then all threads of
+// the workgroup must call this together).
 template <int N, int E, int P, int SIGN, class L, int R, int... Rest>
 __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
                                            const float2 *__restrict__ tw,
@@ -232,10 +255,10 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    lds.sync();
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = lds(t + T * e);
-    __syncthreads();
+    lds.sync();
     fft_stages<N, E, P * R, SIGN, L>(v, t, lds, tw, Radices<Rest...>{});
   }
 }
@@ -437,7 +460,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   const RowLds L{sm.tile + q * M, q & 15};
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
-  __syncthreads();
+  L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
   // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
   // |s1| < 1e-5 (compared in double) gives 0, as the reference's ML step
@@ -490,7 +513,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   const RowLds L{sm.tile + q * M, q & 15};
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
-  __syncthreads();
+  L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
   // u' = (c * u) * s2: the reference's g' = g * s2 with g = c * u stored
   // rounded (src/MaximumLikelihoodDeconvolve.cxx:69-78)
@@ -543,7 +566,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
   stage_table<NT>(tw, gtw, N);
   __syncthreads();
-  const PencilLds L{lds + pl * N};
+  const PencilLds<(T <= 64)> L{lds + pl * N};
   fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename Plan<N, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -610,11 +633,13 @@ const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod", "y_inv",
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
 // Launch shape of the fused kernels; defaults per size, overridable with
-// CLARITY_FAST_TUNE="xe=16,xl=16,xt=8,yp=8,ze=8,zp=1" (x elements per thread
-// for x_fwd/x_mid and for x_last, x tile rows, y pencils per workgroup, z
-// elements per thread, z OTF prefetch).
+// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zp=0"
+// (elements per thread and tile rows of x_fwd, x_mid, x_last; y pencils per
+// workgroup; z elements per thread; z OTF prefetch).
 struct Tune {
-  int xe = 0, xle = 0, xt = 16, yp = 0, ze = 0, zp = -1;
+  int xe[3] = {0, 0, 0};     // x_fwd, x_mid, x_last: elements per thread
+  int xt[3] = {16, 16, 16};  // rows per tile
+  int yp = 0, ze = 0, zp = -1;
 };
 
 bool x_supported(int M) {
@@ -677,15 +702,16 @@ void launch_x(int e, int tp, Kind kind, const Geom &g, const Bufs &b, hipStream_
 }
 
 void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  const int e = (kind == KX_LAST) ? tu.xle : tu.xe;
+  const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1 : 2;
+  const int e = tu.xe[j], tp = tu.xt[j];
   switch (g.nx / 2) {
-    case 64: launch_x<64>(e, tu.xt, kind, g, b, s); break;
-    case 96: launch_x<96>(e, tu.xt, kind, g, b, s); break;
-    case 128: launch_x<128>(e, tu.xt, kind, g, b, s); break;
-    case 192: launch_x<192>(e, tu.xt, kind, g, b, s); break;
-    case 256: launch_x<256>(e, tu.xt, kind, g, b, s); break;
-    case 384: launch_x<384>(e, tu.xt, kind, g, b, s); break;
-    case 512: launch_x<512>(e, tu.xt, kind, g, b, s); break;
+    case 64: launch_x<64>(e, tp, kind, g, b, s); break;
+    case 96: launch_x<96>(e, tp, kind, g, b, s); break;
+    case 128: launch_x<128>(e, tp, kind, g, b, s); break;
+    case 192: launch_x<192>(e, tp, kind, g, b, s); break;
+    case 256: launch_x<256>(e, tp, kind, g, b, s); break;
+    case 384: launch_x<384>(e, tp, kind, g, b, s); break;
+    case 512: launch_x<512>(e, tp, kind, g, b, s); break;
   }
 }
 
@@ -767,9 +793,12 @@ void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipSt
 // Default shapes, then the environment override.
 Tune pick_tune(const Geom &g) {
   Tune t;
-  t.xe = (g.nx / 2 == 256) ? 16 : 8;
-  t.xle = t.xe;
-  t.xt = 16;
+  const int xe = (g.nx / 2 == 256) ? 16 : 8;
+  for (int j = 0; j < 3; j++) {
+    t.xe[j] = xe;
+    t.xt[j] = 16;
+  }
+  if (g.nx / 2 == 512) t.xt[1] = t.xt[2] = 8;  // measured: x_mid, x_last
   t.yp = 4;
   t.ze = (g.nz == 128) ? 8 : 16;
   t.zp = 0;
@@ -777,9 +806,12 @@ Tune pick_tune(const Geom &g) {
     int v;
     const char *p = s;
     while (*p) {
-      if (std::sscanf(p, "xe=%d", &v) == 1) t.xe = v;
-      else if (std::sscanf(p, "xl=%d", &v) == 1) t.xle = v;
-      else if (std::sscanf(p, "xt=%d", &v) == 1) t.xt = v;
+      if (std::sscanf(p, "fe=%d", &v) == 1) t.xe[0] = v;
+      else if (std::sscanf(p, "ft=%d", &v) == 1) t.xt[0] = v;
+      else if (std::sscanf(p, "me=%d", &v) == 1) t.xe[1] = v;
+      else if (std::sscanf(p, "mt=%d", &v) == 1) t.xt[1] = v;
+      else if (std::sscanf(p, "le=%d", &v) == 1) t.xe[2] = v;
+      else if (std::sscanf(p, "lt=%d", &v) == 1) t.xt[2] = v;
       else if (std::sscanf(p, "yp=%d", &v) == 1) t.yp = v;
       else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
       else if (std::sscanf(p, "zp=%d", &v) == 1) t.zp = v;
@@ -801,7 +833,7 @@ class FastRL : public RLEngine {
     g_.hx = g_.nx / 2 + 1;
     g_.sz = (long long)g_.hx * g_.ny;
     tu_ = pick_tune(g_);
-    nparts_ = x_tiles(g_, tu_.xt == 8 ? 8 : 16);
+    nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
   }
   ~FastRL() override {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
