@@ -278,7 +278,9 @@ def test_blind_rl_vs_oracle(clarity_gpu, cfg1):
     assert eo <= max(1e-4, 4 * spread_o)
     assert eh <= max(1e-4, 4 * spread_h)
     assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
-    assert np.all(h >= 0)
+    # h * (O (*) r) is non-negative up to the float32 roundoff of a
+    # near-zero convolution value
+    assert h.min() >= -1e-6 * h.max()
 
 
 def test_idivergence_vs_oracle(clarity_gpu):
