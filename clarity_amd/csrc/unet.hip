@@ -1,0 +1,862 @@
+// Learned warm start: a 3-level 3D U-Net (channels 16/32/64, 3x3x3
+// convolutions, ReLU, skip connections) that maps an observation to an
+// estimate of its 200-iteration Richardson-Lucy result; the estimate then
+// seeds a short RL run.  No counterpart in the reference.
+//
+// Tensors are channels-last bfloat16 ([z][y][x][c], c fastest); weights are
+// float32 masters with bfloat16 copies in two layouts, [tap][co][ci] for the
+// forward pass and [tap][ci][co] for the input gradient.  Every 3x3x3
+// convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16 (float32
+// accumulation): M = voxels, N = output channels, K = 27 taps x input
+// channels in chunks of 16.  The weight gradient is the same GEMM with K =
+// voxels, staged through LDS, reduced over voxel chunks in a fixed order.
+//
+// Network (C = channels):
+//   in(16: obs/mean, sigma_xy/4, sigma_z/8, zeros) -> c1a(16) -> e1(16)
+//   pool -> c2a(32) -> e2(32) -> pool -> c3a(64) -> b3(64)
+//   up(b3)|e2 (96) -> c4a(32) -> d2(32) -> up(d2)|e1 (48) -> c5a(16) -> d1(16)
+//   out = obs/mean + head(d1) (1x1x1, 16 -> 1, float32)
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+
+namespace clarity {
+namespace {
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16;
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+
+struct Vol {
+  int D, H, W;
+  __host__ __device__ long long n() const { return (long long)D * H * W; }
+};
+
+__device__ __forceinline__ void tap_offset(int tap, int &dz, int &dy, int &dx) {
+  dz = tap / 9 - 1;
+  dy = (tap / 3) % 3 - 1;
+  dx = tap % 3 - 1;
+}
+
+// ------------------------------------------------------------------ conv
+
+// Y = relu(conv(X, W) + b).  Block: 4 wavefronts x 16 voxels along x.
+// A fragment (voxel m = lane%16, input channels c0 + 4*(lane/16) + j),
+// B fragment (output channel n = lane%16, same channels), D fragment
+// (voxel 4*(lane/16) + j, output channel lane%16).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256)
+    k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ Wf,
+               const float *__restrict__ bias, bf16 *__restrict__ Y, Vol v) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int x0 = blockIdx.x * 64 + wave * 16;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const int xv = x0 + m;
+  f32x4 acc[COUT / 16];
+#pragma unroll
+  for (int c = 0; c < COUT / 16; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; tap++) {
+    int dz, dy, dx;
+    tap_offset(tap, dz, dy, dx);
+    const int zz = z + dz, yy = y + dy, xx = xv + dx;
+    const bool in = xv < v.W && zz >= 0 && zz < v.D && yy >= 0 && yy < v.H && xx >= 0 &&
+                    xx < v.W;
+    const bf16 *xp = X + (((long long)zz * v.H + yy) * v.W + xx) * CIN + 4 * kq;
+    const bf16 *wp = Wf + ((long long)tap * COUT + m) * CIN + 4 * kq;
+#pragma unroll
+    for (int c0 = 0; c0 < CIN; c0 += 16) {
+      s16x4 a = {0, 0, 0, 0};
+      if (in) a = *reinterpret_cast<const s16x4 *>(xp + c0);
+#pragma unroll
+      for (int ct = 0; ct < COUT / 16; ct++) {
+        const s16x4 b = *reinterpret_cast<const s16x4 *>(wp + ct * 16 * CIN + c0);
+        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+      }
+    }
+  }
+  const long long rowbase = ((long long)z * v.H + y) * v.W;
+#pragma unroll
+  for (int ct = 0; ct < COUT / 16; ct++) {
+    const int co = ct * 16 + m;
+    const float bb = bias[co];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int xo = x0 + 4 * kq + j;
+      if (xo < v.W) {
+        float r = acc[ct][j] + bb;
+        r = r > 0.f ? r : 0.f;
+        Y[(rowbase + xo) * COUT + co] = f2bf(r);
+      }
+    }
+  }
+}
+
+// GX = conv^T(G * [Y > 0], W): input gradient of a ReLU convolution.
+// Same GEMM shape with taps mirrored and W laid out [tap][ci][co].
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256)
+    k_conv_bwd_data(const bf16 *__restrict__ G, const bf16 *__restrict__ Yo,
+                    const bf16 *__restrict__ Wb, bf16 *__restrict__ GX, Vol v) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int x0 = blockIdx.x * 64 + wave * 16;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const int xv = x0 + m;
+  f32x4 acc[CIN / 16];
+#pragma unroll
+  for (int c = 0; c < CIN / 16; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; tap++) {
+    int dz, dy, dx;
+    tap_offset(tap, dz, dy, dx);
+    // output voxel p = q - off reads input q: GX[q] += W[tap] . G[q - off]
+    const int zz = z - dz, yy = y - dy, xx = xv - dx;
+    const bool in = xv < v.W && zz >= 0 && zz < v.D && yy >= 0 && yy < v.H && xx >= 0 &&
+                    xx < v.W;
+    const long long off = (((long long)zz * v.H + yy) * v.W + xx) * COUT + 4 * kq;
+    const bf16 *wp = Wb + ((long long)tap * CIN + m) * COUT + 4 * kq;
+#pragma unroll
+    for (int c0 = 0; c0 < COUT; c0 += 16) {
+      s16x4 a = {0, 0, 0, 0};
+      if (in) {
+        const s16x4 g = *reinterpret_cast<const s16x4 *>(G + off + c0);
+        const s16x4 o = *reinterpret_cast<const s16x4 *>(Yo + off + c0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) a[j] = (o[j] > 0) ? g[j] : (short)0;  // bf16 > 0
+      }
+#pragma unroll
+      for (int it = 0; it < CIN / 16; it++) {
+        const s16x4 b = *reinterpret_cast<const s16x4 *>(wp + it * 16 * COUT + c0);
+        acc[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[it], 0, 0, 0);
+      }
+    }
+  }
+  const long long rowbase = ((long long)z * v.H + y) * v.W;
+#pragma unroll
+  for (int it = 0; it < CIN / 16; it++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int xo = x0 + 4 * kq + j;
+      if (xo < v.W) GX[(rowbase + xo) * CIN + it * 16 + m] = f2bf(acc[it][j]);
+    }
+  }
+}
+
+// Weight gradient of one tap over one chunk of voxels:
+// P[chunk][tap][co][ci] = sum_v (G * [Y > 0])[v][co] * X[v + off][ci].
+// 64 voxels per step are staged in LDS (masked G and shifted X); wavefront
+// w takes voxels 16w..16w+15 of each step as the K of its MFMAs; the four
+// wavefronts' sums are added at the end.
+constexpr int kWChunk = 8192;
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256)
+    k_conv_bwd_w(const bf16 *__restrict__ G, const bf16 *__restrict__ Yo,
+                 const bf16 *__restrict__ X, float *__restrict__ P, Vol v) {
+  constexpr int NT = (COUT / 16) * (CIN / 16);
+  __shared__ bf16 sg[64 * COUT];
+  __shared__ bf16 sx[64 * CIN];
+  __shared__ float red[4][16 * 16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const int tap = blockIdx.y;
+  int dz, dy, dx;
+  tap_offset(tap, dz, dy, dx);
+  const long long vb = (long long)blockIdx.x * kWChunk;
+  const long long ve = vb + kWChunk < v.n() ? vb + kWChunk : v.n();
+  f32x4 acc[NT];
+#pragma unroll
+  for (int i = 0; i < NT; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (long long s0 = vb; s0 < ve; s0 += 64) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * COUT; i += 256) {
+      const long long p = s0 + i / COUT;
+      bf16 gv = 0;
+      if (p < ve) {
+        const long long o = p * COUT + i % COUT;
+        if ((short)Yo[o] > 0) gv = G[o];
+      }
+      sg[i] = gv;
+    }
+    for (int i = threadIdx.x; i < 64 * CIN; i += 256) {
+      const long long p = s0 + i / CIN;
+      bf16 xv = 0;
+      if (p < ve) {
+        const int x = (int)(p % v.W);
+        const long long t = p / v.W;
+        const int y = (int)(t % v.H), z = (int)(t / v.H);
+        const int zz = z + dz, yy = y + dy, xx = x + dx;
+        if (zz >= 0 && zz < v.D && yy >= 0 && yy < v.H && xx >= 0 && xx < v.W)
+          xv = X[(((long long)zz * v.H + yy) * v.W + xx) * CIN + i % CIN];
+      }
+      sx[i] = xv;
+    }
+    __syncthreads();
+    const int vr = wave * 16 + 4 * kq;  // the lane's 4 voxels
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++) {
+      s16x4 a;
+#pragma unroll
+      for (int j = 0; j < 4; j++) a[j] = (short)sg[(vr + j) * COUT + ct * 16 + n];
+#pragma unroll
+      for (int it = 0; it < CIN / 16; it++) {
+        s16x4 b;
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = (short)sx[(vr + j) * CIN + it * 16 + n];
+        acc[ct * (CIN / 16) + it] =
+            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct * (CIN / 16) + it], 0, 0, 0);
+      }
+    }
+  }
+  // D fragment: row (co) 4*kq + j, column (ci) n
+  float *out = P + ((long long)blockIdx.x * 27 + tap) * COUT * CIN;
+#pragma unroll
+  for (int i = 0; i < NT; i++) {
+    const int ct = i / (CIN / 16), it = i % (CIN / 16);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; j++) red[wave][(4 * kq + j) * 16 + n] = acc[i][j];
+    __syncthreads();
+    if (threadIdx.x < 256) {
+      const int r = threadIdx.x / 16, c = threadIdx.x % 16;
+      const float sum = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                        red[3][threadIdx.x];
+      out[(ct * 16 + r) * CIN + it * 16 + c] = sum;
+    }
+  }
+}
+
+// Bias gradient partials: Pb[chunk][co] = sum_v (G * [Y > 0])[v][co].
+template <int COUT>
+__global__ void __launch_bounds__(256)
+    k_bias_grad(const bf16 *__restrict__ G, const bf16 *__restrict__ Yo,
+                float *__restrict__ Pb, long long nvox) {
+  __shared__ float sh[256];
+  const long long vb = (long long)blockIdx.x * kWChunk;
+  const long long ve = vb + kWChunk < nvox ? vb + kWChunk : nvox;
+  const int co = threadIdx.x % COUT;
+  const int lanes = 256 / COUT;
+  float acc = 0.f;
+  for (long long p = vb + threadIdx.x / COUT; p < ve; p += lanes) {
+    const long long o = p * COUT + co;
+    if ((short)Yo[o] > 0) acc += bf2f(G[o]);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < COUT) {
+    float s = 0.f;
+    for (int k = 0; k < lanes; k++) s += sh[k * COUT + threadIdx.x];
+    Pb[(long long)blockIdx.x * COUT + threadIdx.x] = s;
+  }
+}
+
+// out[i] = sum_c P[c][i] for c < nchunks (fixed order)
+__global__ void k_reduce_chunks(const float *__restrict__ P, int nchunks, long long n,
+                                float *__restrict__ out) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < nchunks; c++) s += P[(long long)c * n + i];
+    out[i] = s;
+  }
+}
+
+// ----------------------------------------------------- pool / up / concat
+
+#define VOX_LOOP(i, n)                                                      \
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
+       i += (long long)gridDim.x * blockDim.x)
+
+// 2x2x2 average pooling; v is the fine volume
+__global__ void k_pool(const bf16 *__restrict__ X, bf16 *__restrict__ Y, Vol v, int C) {
+  const int D2 = v.D / 2, H2 = v.H / 2, W2 = v.W / 2;
+  const long long n = (long long)D2 * H2 * W2 * C;
+  VOX_LOOP(i, n) {
+    const int c = (int)(i % C);
+    long long p = i / C;
+    const int x = (int)(p % W2);
+    p /= W2;
+    const int y = (int)(p % H2), z = (int)(p / H2);
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      s += bf2f(X[(((long long)zz * v.H + yy) * v.W + xx) * C + c]);
+    }
+    Y[i] = f2bf(0.125f * s);
+  }
+}
+
+// GX (fine) = GX + pool^T(GP) = GX + GP[parent] / 8
+__global__ void k_pool_bwd_add(const bf16 *__restrict__ GP, bf16 *__restrict__ GX, Vol v,
+                               int C) {
+  const long long n = v.n() * C;
+  VOX_LOOP(i, n) {
+    const int c = (int)(i % C);
+    long long p = i / C;
+    const int x = (int)(p % v.W);
+    p /= v.W;
+    const int y = (int)(p % v.H), z = (int)(p / v.H);
+    const long long q = (((long long)(z / 2) * (v.H / 2) + y / 2) * (v.W / 2) + x / 2) * C + c;
+    GX[i] = f2bf(bf2f(GX[i]) + 0.125f * bf2f(GP[q]));
+  }
+}
+
+// Y (fine, Ctot channels) channels [0, C1) = nearest upsampling of X
+// (coarse, C1 channels); channels [C1, Ctot) = S (fine, Ctot - C1 channels)
+__global__ void k_up_concat(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+                            bf16 *__restrict__ Y, Vol v, int C1, int Ctot) {
+  const int CS = Ctot - C1;
+  const long long n = v.n() * Ctot;
+  VOX_LOOP(i, n) {
+    const int c = (int)(i % Ctot);
+    long long p = i / Ctot;
+    const long long pv = p;
+    const int x = (int)(p % v.W);
+    p /= v.W;
+    const int y = (int)(p % v.H), z = (int)(p / v.H);
+    if (c < C1) {
+      Y[i] = X[(((long long)(z / 2) * (v.H / 2) + y / 2) * (v.W / 2) + x / 2) * C1 + c];
+    } else {
+      Y[i] = S[pv * CS + (c - C1)];
+    }
+  }
+}
+
+// GXc (coarse, C1) = sum over the 8 children of G channels [0, C1);
+// GS (fine, Ctot - C1) = G channels [C1, Ctot)
+__global__ void k_up_concat_bwd(const bf16 *__restrict__ G, bf16 *__restrict__ GXc,
+                                bf16 *__restrict__ GS, Vol v, int C1, int Ctot) {
+  const int CS = Ctot - C1;
+  const int D2 = v.D / 2, H2 = v.H / 2, W2 = v.W / 2;
+  const long long nc = (long long)D2 * H2 * W2 * C1;
+  VOX_LOOP(i, nc) {
+    const int c = (int)(i % C1);
+    long long p = i / C1;
+    const int x = (int)(p % W2);
+    p /= W2;
+    const int y = (int)(p % H2), z = (int)(p / H2);
+    float s = 0.f;
+    for (int k = 0; k < 8; k++) {
+      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      s += bf2f(G[(((long long)zz * v.H + yy) * v.W + xx) * Ctot + c]);
+    }
+    GXc[i] = f2bf(s);
+  }
+  const long long nf = v.n() * CS;
+  VOX_LOOP(i, nf) {
+    const int c = (int)(i % CS);
+    const long long p = i / CS;
+    GS[i] = G[p * Ctot + C1 + c];
+  }
+}
+
+// ------------------------------------------------------ input, head, loss
+
+// in[v] = (obs/mean, sigma_xy/4, sigma_z/8, 0...) in 16 channels
+__global__ void k_input(const float *__restrict__ obs, const float *__restrict__ inv_mean,
+                        float sxy, float sz, bf16 *__restrict__ in, long long n) {
+  const float im = inv_mean[0];
+  VOX_LOOP(i, n) {
+    bf16 *p = in + i * 16;
+    p[0] = f2bf(obs[i] * im);
+    p[1] = f2bf(sxy * 0.25f);
+    p[2] = f2bf(sz * 0.125f);
+    for (int c = 3; c < 16; c++) p[c] = 0;
+  }
+}
+
+// out[v] = obs[v]/mean + b + sum_c w[c] d1[v][c]   (scaled by mean when
+// `rescale`, i.e. in the observation's units)
+__global__ void k_head(const bf16 *__restrict__ d1, const float *__restrict__ w,
+                       const float *__restrict__ obs, const float *__restrict__ inv_mean,
+                       float *__restrict__ out, long long n, int rescale) {
+  const float im = inv_mean[0];
+  const float b = w[16];
+  VOX_LOOP(i, n) {
+    float s = b;
+    for (int c = 0; c < 16; c++) s += w[c] * bf2f(d1[i * 16 + c]);
+    float r = obs[i] * im + s;
+    if (rescale) r = (r > 0.f ? r : 0.f) / im;
+    out[i] = r;
+  }
+}
+
+// Loss L = mean((out - t/mean)^2) (fp64 partials); gout = 2 (out - t/mean)/n
+// and the head's gradients: gd1[v][c] = gout w[c], partial dw, db.
+__global__ void __launch_bounds__(256)
+    k_loss_head_bwd(const float *__restrict__ out, const float *__restrict__ tgt,
+                    const float *__restrict__ inv_mean, const bf16 *__restrict__ d1,
+                    const float *__restrict__ w, bf16 *__restrict__ gd1,
+                    double *__restrict__ lossp, float *__restrict__ whp, long long n) {
+  __shared__ double sl[256];
+  __shared__ float sw[17][256];
+  const float im = inv_mean[0];
+  const float inv_n = 1.0f / (float)n;
+  double lacc = 0.0;
+  float wacc[17];
+  for (int c = 0; c < 17; c++) wacc[c] = 0.f;
+  VOX_LOOP(i, n) {
+    const float d = out[i] - tgt[i] * im;
+    lacc += (double)d * d;
+    const float g = 2.f * d * inv_n;
+    for (int c = 0; c < 16; c++) {
+      const float a = bf2f(d1[i * 16 + c]);
+      wacc[c] += g * a;
+      gd1[i * 16 + c] = f2bf(g * w[c]);
+    }
+    wacc[16] += g;
+  }
+  sl[threadIdx.x] = lacc;
+  for (int c = 0; c < 17; c++) sw[c][threadIdx.x] = wacc[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sl[threadIdx.x] += sl[threadIdx.x + s];
+      for (int c = 0; c < 17; c++) sw[c][threadIdx.x] += sw[c][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    lossp[blockIdx.x] = sl[0];
+    for (int c = 0; c < 17; c++) whp[blockIdx.x * 17 + c] = sw[c][0];
+  }
+}
+
+// Adam on float32 masters; refreshes the bf16 copies of conv weights in
+// both layouts ([tap][co][ci] and [tap][ci][co]) when cin > 0.
+__global__ void k_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ s,
+                       const float *__restrict__ g, long long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, bf16 *__restrict__ wf,
+                       bf16 *__restrict__ wb, int cin, int cout) {
+  VOX_LOOP(i, n) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float si = b2 * s[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    s[i] = si;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(si / c2) + eps);
+    p[i] = pi;
+    if (cin > 0) {
+      // i indexes [tap][co][ci]
+      const int ci = (int)(i % cin);
+      const long long t = i / cin;
+      const int co = (int)(t % cout), tap = (int)(t / cout);
+      wf[i] = f2bf(pi);
+      wb[((long long)tap * cin + ci) * cout + co] = f2bf(pi);
+    }
+  }
+}
+
+// a[1] = n / a[0]: the reciprocal of the mean from the sum
+__global__ void k_recip_mean(float *a, float n) { a[1] = n / a[0]; }
+
+unsigned grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 65535LL * 8) b = 65535LL * 8;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// ------------------------------------------------------------ the network
+
+struct ConvSpec {
+  int cin, cout;
+};
+// c1a c1b c2a c2b c3a c3b c4a c4b c5a c5b
+constexpr ConvSpec kConv[10] = {{16, 16}, {16, 16}, {16, 32}, {32, 32}, {32, 64},
+                                {64, 64}, {96, 32}, {32, 32}, {48, 16}, {16, 16}};
+constexpr int kLevel[10] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
+
+template <int CIN, int COUT>
+void conv_fwd_t(const bf16 *X, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
+                hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT>), dim3((v.W + 63) / 64, v.H, v.D), dim3(256), 0,
+                     s, X, Wf, b, Y, v);
+}
+template <int CIN, int COUT>
+void conv_bwd_t(const bf16 *G, const bf16 *Yo, const bf16 *X, const bf16 *Wb, bf16 *GX,
+                float *P, float *Pb, Vol v, hipStream_t s) {
+  if (GX)
+    hipLaunchKernelGGL((k_conv_bwd_data<CIN, COUT>), dim3((v.W + 63) / 64, v.H, v.D),
+                       dim3(256), 0, s, G, Yo, Wb, GX, v);
+  const int nch = (int)((v.n() + kWChunk - 1) / kWChunk);
+  hipLaunchKernelGGL((k_conv_bwd_w<CIN, COUT>), dim3(nch, 27), dim3(256), 0, s, G, Yo, X,
+                     P, v);
+  hipLaunchKernelGGL((k_bias_grad<COUT>), dim3(nch), dim3(256), 0, s, G, Yo, Pb, v.n());
+}
+
+#define CONV_SWITCH(I, CALL)                          \
+  switch (I) {                                        \
+    case 0: case 1: case 9: { constexpr int A = 16, B = 16; CALL; } break; \
+    case 2: { constexpr int A = 16, B = 32; CALL; } break;                 \
+    case 3: case 7: { constexpr int A = 32, B = 32; CALL; } break;         \
+    case 4: { constexpr int A = 32, B = 64; CALL; } break;                 \
+    case 5: { constexpr int A = 64, B = 64; CALL; } break;                 \
+    case 6: { constexpr int A = 96, B = 32; CALL; } break;                 \
+    case 8: { constexpr int A = 48, B = 16; CALL; } break;                 \
+  }
+
+}  // namespace
+
+class UNet {
+ public:
+  // parameter blocks: 10 conv weights [27][co][ci], 10 biases, head (17)
+  std::vector<long long> off_, len_;
+  long long nparam_ = 0;
+  DevBuf p_, m_, s_, g_;  // float32 masters, Adam moments, gradients
+  DevBuf wf_[10], wb_[10];
+  long long step_ = 0;
+  // activation buffers for the current shape
+  Vol v0_{0, 0, 0};
+  DevBuf in_, act_[10], pool1_, pool2_, cat2_, cat1_, out_, invm_, parts_;
+  DevBuf gact_[10], gcat2_, gcat1_, gskip1_, gskip2_, gp1_, gp2_, gd1_, wpart_, bpart_;
+  DevBuf lossp_, whp_, sumbuf_;
+
+  ClarityResult_t init(unsigned seed) {
+    long long o = 0;
+    for (int i = 0; i < 10; i++) {
+      off_.push_back(o);
+      len_.push_back(27LL * kConv[i].cin * kConv[i].cout);
+      o += len_.back();
+    }
+    for (int i = 0; i < 10; i++) {
+      off_.push_back(o);
+      len_.push_back(kConv[i].cout);
+      o += kConv[i].cout;
+    }
+    off_.push_back(o);
+    len_.push_back(17);
+    o += 17;
+    nparam_ = o;
+    std::vector<float> h(nparam_, 0.f);
+    std::mt19937 rng(seed);
+    for (int i = 0; i < 10; i++) {
+      // He initialisation over the real fan-in (layer 0 has 3 live channels)
+      const int fan = 27 * (i == 0 ? 3 : kConv[i].cin);
+      std::normal_distribution<float> nd(0.f, std::sqrt(2.f / fan));
+      for (long long k = 0; k < len_[i]; k++) {
+        const int ci = (int)(k % kConv[i].cin);
+        h[off_[i] + k] = (i == 0 && ci >= 3) ? 0.f : nd(rng);
+      }
+    }
+    // head starts near zero: the untrained network returns the observation
+    std::normal_distribution<float> hd(0.f, 1e-3f);
+    for (int c = 0; c < 16; c++) h[off_[20] + c] = hd(rng);
+    CL_TRY(p_.alloc(sizeof(float) * nparam_));
+    CL_TRY(m_.alloc(sizeof(float) * nparam_));
+    CL_TRY(s_.alloc(sizeof(float) * nparam_));
+    CL_TRY(g_.alloc(sizeof(float) * nparam_));
+    for (int i = 0; i < 10; i++) {
+      CL_TRY(wf_[i].alloc(sizeof(bf16) * len_[i]));
+      CL_TRY(wb_[i].alloc(sizeof(bf16) * len_[i]));
+    }
+    return set_params(h.data());
+  }
+
+  ClarityResult_t set_params(const float *host) {
+    hipStream_t s = rt().stream;
+    CL_HIP(hipMemcpyAsync(p_.p, host, sizeof(float) * nparam_, hipMemcpyHostToDevice, s));
+    CL_HIP(hipMemsetAsync(m_.p, 0, sizeof(float) * nparam_, s));
+    CL_HIP(hipMemsetAsync(s_.p, 0, sizeof(float) * nparam_, s));
+    step_ = 0;
+    // bf16 copies: an Adam step with lr 0 rewrites them from the masters
+    CL_HIP(hipMemsetAsync(g_.p, 0, sizeof(float) * nparam_, s));
+    for (int i = 0; i < 10; i++)
+      hipLaunchKernelGGL(k_adam, dim3(grid_for(len_[i])), dim3(256), 0, s, p_.f() + off_[i],
+                         m_.f() + off_[i], s_.f() + off_[i], g_.f() + off_[i], len_[i], 0.f,
+                         0.f, 0.f, 1.f, 1.f, 1.f, (bf16 *)wf_[i].p, (bf16 *)wb_[i].p,
+                         kConv[i].cin, kConv[i].cout);
+    CL_TRY(launch_check());
+    CL_HIP(hipStreamSynchronize(s));
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t shape(Vol v, bool train) {
+    if (v.D % 4 || v.H % 4 || v.W % 4 || v.D < 4 || v.H < 4 || v.W < 4)
+      return CLARITY_INVALID_ARGUMENT;
+    if (v.D == v0_.D && v.H == v0_.H && v.W == v0_.W && (!train || gd1_.p)) return CLARITY_SUCCESS;
+    v0_ = v;
+    const long long n[3] = {v.n(), v.n() / 8, v.n() / 64};
+    CL_TRY(in_.alloc(sizeof(bf16) * n[0] * 16));
+    for (int i = 0; i < 10; i++)
+      CL_TRY(act_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
+    CL_TRY(pool1_.alloc(sizeof(bf16) * n[1] * 16));
+    CL_TRY(pool2_.alloc(sizeof(bf16) * n[2] * 32));
+    CL_TRY(cat2_.alloc(sizeof(bf16) * n[1] * 96));
+    CL_TRY(cat1_.alloc(sizeof(bf16) * n[0] * 48));
+    CL_TRY(out_.alloc(sizeof(float) * n[0]));
+    CL_TRY(invm_.alloc(sizeof(float) * 4));
+    CL_TRY(parts_.alloc(sizeof(double) * kParts));
+    if (train) {
+      for (int i = 0; i < 10; i++)
+        CL_TRY(gact_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
+      CL_TRY(gcat2_.alloc(sizeof(bf16) * n[1] * 96));
+      CL_TRY(gcat1_.alloc(sizeof(bf16) * n[0] * 48));
+      CL_TRY(gskip1_.alloc(sizeof(bf16) * n[0] * 16));
+      CL_TRY(gskip2_.alloc(sizeof(bf16) * n[1] * 32));
+      CL_TRY(gp1_.alloc(sizeof(bf16) * n[1] * 16));
+      CL_TRY(gp2_.alloc(sizeof(bf16) * n[2] * 32));
+      CL_TRY(gd1_.alloc(sizeof(bf16) * n[0] * 16));
+      const long long nch = (n[0] + kWChunk - 1) / kWChunk;
+      CL_TRY(wpart_.alloc(sizeof(float) * nch * 27 * 64 * 64));
+      CL_TRY(bpart_.alloc(sizeof(float) * nch * 64));
+      CL_TRY(lossp_.alloc(sizeof(double) * 1024));
+      CL_TRY(whp_.alloc(sizeof(float) * 1024 * 17));
+      CL_TRY(sumbuf_.alloc(sizeof(float) * 32));
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  // forward on a device observation; leaves activations for backward
+  ClarityResult_t forward(const float *obs_d, float sxy, float sz, hipStream_t s) {
+    const Vol v = v0_;
+    const Vol v1{v.D / 2, v.H / 2, v.W / 2}, v2{v.D / 4, v.H / 4, v.W / 4};
+    const Vol lv[3] = {v, v1, v2};
+    // 1 / mean(obs)
+    CL_TRY(k_sum_partials(obs_d, v.n(), parts_.d(), kParts, s));
+    CL_TRY(k_finish_sum(parts_.d(), kParts, invm_.f(), nullptr, nullptr, s));
+    hipLaunchKernelGGL(k_recip_mean, dim3(1), dim3(1), 0, s, invm_.f(), (float)v.n());
+    hipLaunchKernelGGL(k_input, dim3(grid_for(v.n())), dim3(256), 0, s, obs_d, invm_.f() + 1,
+                       sxy, sz, (bf16 *)in_.p, v.n());
+    const bf16 *src[10];
+    src[0] = (bf16 *)in_.p;
+    src[1] = (bf16 *)act_[0].p;
+    src[2] = (bf16 *)pool1_.p;
+    src[3] = (bf16 *)act_[2].p;
+    src[4] = (bf16 *)pool2_.p;
+    src[5] = (bf16 *)act_[4].p;
+    src[6] = (bf16 *)cat2_.p;
+    src[7] = (bf16 *)act_[6].p;
+    src[8] = (bf16 *)cat1_.p;
+    src[9] = (bf16 *)act_[8].p;
+    for (int i = 0; i < 10; i++) {
+      if (i == 2)
+        hipLaunchKernelGGL(k_pool, dim3(grid_for(v1.n() * 16)), dim3(256), 0, s,
+                           (bf16 *)act_[1].p, (bf16 *)pool1_.p, v, 16);
+      if (i == 4)
+        hipLaunchKernelGGL(k_pool, dim3(grid_for(v2.n() * 32)), dim3(256), 0, s,
+                           (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1, 32);
+      if (i == 6)
+        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(v1.n() * 96)), dim3(256), 0, s,
+                           (bf16 *)act_[5].p, (bf16 *)act_[3].p, (bf16 *)cat2_.p, v1, 64, 96);
+      if (i == 8)
+        hipLaunchKernelGGL(k_up_concat, dim3(grid_for(v.n() * 48)), dim3(256), 0, s,
+                           (bf16 *)act_[7].p, (bf16 *)act_[1].p, (bf16 *)cat1_.p, v, 32, 48);
+      const bf16 *X = src[i];
+      bf16 *Y = (bf16 *)act_[i].p;
+      const float *b = p_.f() + off_[10 + i];
+      const bf16 *Wf = (bf16 *)wf_[i].p;
+      const Vol vv = lv[kLevel[i]];
+      CONV_SWITCH(i, (conv_fwd_t<A, B>(X, Wf, b, Y, vv, s)));
+    }
+    CL_TRY(launch_check());
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t head(const float *obs_d, float *out_d, bool rescale, hipStream_t s) {
+    hipLaunchKernelGGL(k_head, dim3(grid_for(v0_.n())), dim3(256), 0, s,
+                       (bf16 *)act_[9].p, p_.f() + off_[20], obs_d, invm_.f() + 1, out_d,
+                       v0_.n(), rescale ? 1 : 0);
+    return launch_check();
+  }
+
+  // loss and gradients for target tgt_d; Adam step when lr > 0
+  ClarityResult_t backward(const float *tgt_d, float lr, float *loss_h, hipStream_t s) {
+    const Vol v = v0_;
+    const Vol v1{v.D / 2, v.H / 2, v.W / 2}, v2{v.D / 4, v.H / 4, v.W / 4};
+    const Vol lv[3] = {v, v1, v2};
+    CL_HIP(hipMemsetAsync(g_.p, 0, sizeof(float) * nparam_, s));
+    const int nb = 1024;
+    hipLaunchKernelGGL(k_loss_head_bwd, dim3(nb), dim3(256), 0, s, out_.f(), tgt_d,
+                       invm_.f() + 1, (bf16 *)act_[9].p, p_.f() + off_[20],
+                       (bf16 *)gd1_.p, lossp_.d(), whp_.f(), v.n());
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(64), 0, s, whp_.f(), nb, 17LL,
+                       g_.f() + off_[20]);
+    // input of each conv, gradient buffers of each conv's output
+    const bf16 *xin[10] = {(bf16 *)in_.p,    (bf16 *)act_[0].p, (bf16 *)pool1_.p,
+                           (bf16 *)act_[2].p, (bf16 *)pool2_.p,  (bf16 *)act_[4].p,
+                           (bf16 *)cat2_.p,   (bf16 *)act_[6].p, (bf16 *)cat1_.p,
+                           (bf16 *)act_[8].p};
+    bf16 *gout[10];
+    for (int i = 0; i < 9; i++) gout[i] = (bf16 *)gact_[i].p;
+    gout[9] = (bf16 *)gd1_.p;
+    bf16 *gin[10] = {nullptr,          gout[0], (bf16 *)gp1_.p, gout[2], (bf16 *)gp2_.p,
+                     gout[4],          (bf16 *)gcat2_.p, gout[6], (bf16 *)gcat1_.p, gout[8]};
+    for (int i = 9; i >= 0; i--) {
+      const Vol vv = lv[kLevel[i]];
+      const int nch = (int)((vv.n() + kWChunk - 1) / kWChunk);
+      const bf16 *Wb = (bf16 *)wb_[i].p;
+      CONV_SWITCH(i, (conv_bwd_t<A, B>(gout[i], (bf16 *)act_[i].p, xin[i], Wb, gin[i],
+                                       wpart_.f(), bpart_.f(), vv, s)));
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(grid_for(len_[i])), dim3(256), 0, s,
+                         wpart_.f(), nch, len_[i], g_.f() + off_[i]);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(1), dim3(64), 0, s, bpart_.f(), nch,
+                         (long long)kConv[i].cout, g_.f() + off_[10 + i]);
+      // route gradients through concat / upsampling / pooling
+      if (i == 8)  // cat1 = up(d2) | e1
+        hipLaunchKernelGGL(k_up_concat_bwd, dim3(grid_for(v.n() * 32)), dim3(256), 0, s,
+                           (bf16 *)gcat1_.p, gout[7], (bf16 *)gskip1_.p, v, 32, 48);
+      if (i == 6)  // cat2 = up(b3) | e2
+        hipLaunchKernelGGL(k_up_concat_bwd, dim3(grid_for(v1.n() * 64)), dim3(256), 0, s,
+                           (bf16 *)gcat2_.p, gout[5], (bf16 *)gskip2_.p, v1, 64, 96);
+      if (i == 4) {  // e2 feeds pool2 and the skip: sum both gradients
+        CL_HIP(hipMemcpyAsync(gout[3], gskip2_.p, sizeof(bf16) * v1.n() * 32,
+                              hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_pool_bwd_add, dim3(grid_for(v1.n() * 32)), dim3(256), 0, s,
+                           (bf16 *)gp2_.p, gout[3], v1, 32);
+      }
+      if (i == 2) {
+        CL_HIP(hipMemcpyAsync(gout[1], gskip1_.p, sizeof(bf16) * v.n() * 16,
+                              hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_pool_bwd_add, dim3(grid_for(v.n() * 16)), dim3(256), 0, s,
+                           (bf16 *)gp1_.p, gout[1], v, 16);
+      }
+    }
+    CL_TRY(launch_check());
+    std::vector<double> lp(nb);
+    CL_HIP(hipMemcpyAsync(lp.data(), lossp_.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    if (lr > 0.f) {
+      step_++;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - std::pow(b1, (float)step_), c2 = 1.f - std::pow(b2, (float)step_);
+      for (int i = 0; i < 21; i++) {
+        const int cin = i < 10 ? kConv[i].cin : 0, cout = i < 10 ? kConv[i].cout : 0;
+        hipLaunchKernelGGL(k_adam, dim3(grid_for(len_[i])), dim3(256), 0, s, p_.f() + off_[i],
+                           m_.f() + off_[i], s_.f() + off_[i], g_.f() + off_[i], len_[i], lr,
+                           b1, b2, 1e-8f, c1, c2, i < 10 ? (bf16 *)wf_[i].p : nullptr,
+                           i < 10 ? (bf16 *)wb_[i].p : nullptr, cin, cout);
+      }
+      CL_TRY(launch_check());
+    }
+    CL_HIP(hipStreamSynchronize(s));
+    double l = 0.0;
+    for (double x : lp) l += x;
+    if (loss_h) *loss_h = (float)(l / (double)v.n());
+    return CLARITY_SUCCESS;
+  }
+};
+
+}  // namespace clarity
+
+using namespace clarity;
+
+struct ClarityEx_UNet_t {
+  UNet net;
+};
+
+namespace {
+
+#define API_LOCK std::lock_guard<std::recursive_mutex> lk_(rt().mu)
+
+ClarityResult_t unet_run(ClarityEx_UNet h, const float *obs, const float *tgt,
+                         Clarity_Dim3 d, float sxy, float sz, float lr, float *out,
+                         float *loss, float *grads, int on_device) {
+  if (!h || !obs || !dims_ok(d)) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  UNet &n = h->net;
+  const bool train = tgt != nullptr;
+  const Vol v{d.z, d.y, d.x};
+  CL_TRY(n.shape(v, train));
+  hipStream_t s = rt().stream;
+  const long long nv = v.n();
+  DevBuf a, b, o;
+  const float *obs_d = obs, *tgt_d = tgt;
+  if (!on_device) {
+    CL_TRY(a.alloc(sizeof(float) * nv));
+    CL_HIP(hipMemcpyAsync(a.p, obs, sizeof(float) * nv, hipMemcpyHostToDevice, s));
+    obs_d = a.f();
+    if (train) {
+      CL_TRY(b.alloc(sizeof(float) * nv));
+      CL_HIP(hipMemcpyAsync(b.p, tgt, sizeof(float) * nv, hipMemcpyHostToDevice, s));
+      tgt_d = b.f();
+    }
+  }
+  CL_TRY(n.forward(obs_d, sxy, sz, s));
+  if (train) {
+    CL_TRY(n.head(obs_d, n.out_.f(), false, s));
+    CL_TRY(n.backward(tgt_d, lr, loss, s));
+    if (grads)
+      CL_HIP(hipMemcpy(grads, n.g_.p, sizeof(float) * n.nparam_, hipMemcpyDeviceToHost));
+    return CLARITY_SUCCESS;
+  }
+  if (on_device) {
+    CL_TRY(n.head(obs_d, out, true, s));
+  } else {
+    CL_TRY(o.alloc(sizeof(float) * nv));
+    CL_TRY(n.head(obs_d, o.f(), true, s));
+    CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * nv, hipMemcpyDeviceToHost, s));
+  }
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+ClarityResult_t ClarityEx_UNetCreate(ClarityEx_UNet *net, unsigned seed) {
+  API_LOCK;
+  if (!net) return CLARITY_INVALID_ARGUMENT;
+  *net = nullptr;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  std::unique_ptr<ClarityEx_UNet_t> h(new ClarityEx_UNet_t);
+  CL_TRY(h->net.init(seed));
+  *net = h.release();
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t ClarityEx_UNetDestroy(ClarityEx_UNet net) {
+  API_LOCK;
+  if (!net) return CLARITY_INVALID_ARGUMENT;
+  if (rt().stream) (void)hipStreamSynchronize(rt().stream);
+  delete net;
+  return CLARITY_SUCCESS;
+}
+
+int64_t ClarityEx_UNetNumParams(ClarityEx_UNet net) { return net ? net->net.nparam_ : 0; }
+
+ClarityResult_t ClarityEx_UNetGetParams(ClarityEx_UNet net, float *params) {
+  API_LOCK;
+  if (!net || !params) return CLARITY_INVALID_ARGUMENT;
+  CL_HIP(hipStreamSynchronize(rt().stream));
+  CL_HIP(hipMemcpy(params, net->net.p_.p, sizeof(float) * net->net.nparam_,
+                   hipMemcpyDeviceToHost));
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t ClarityEx_UNetSetParams(ClarityEx_UNet net, const float *params) {
+  API_LOCK;
+  if (!net || !params) return CLARITY_INVALID_ARGUMENT;
+  return net->net.set_params(params);
+}
+
+ClarityResult_t ClarityEx_UNetPredict(ClarityEx_UNet net, const float *obs,
+                                      Clarity_Dim3 dims, float sigmaXY, float sigmaZ,
+                                      float *out, int onDevice) {
+  API_LOCK;
+  if (!out) return CLARITY_INVALID_ARGUMENT;
+  return unet_run(net, obs, nullptr, dims, sigmaXY, sigmaZ, 0.f, out, nullptr, nullptr,
+                  onDevice);
+}
+
+ClarityResult_t ClarityEx_UNetTrainStep(ClarityEx_UNet net, const float *obs,
+                                        const float *target, Clarity_Dim3 dims,
+                                        float sigmaXY, float sigmaZ, float learningRate,
+                                        float *loss, float *grads, int onDevice) {
+  API_LOCK;
+  if (!target || learningRate < 0.f) return CLARITY_INVALID_ARGUMENT;
+  return unet_run(net, obs, target, dims, sigmaXY, sigmaZ, learningRate, nullptr, loss,
+                  grads, onDevice);
+}
+
+}  // extern "C"

@@ -113,6 +113,39 @@ C_FUNC_DEF int ClarityEx_RLPlanTimings(ClarityEx_RLPlan plan, int *count,
  * pipeline) or "generic". */
 C_FUNC_DEF const char *ClarityEx_RLPlanEngine(ClarityEx_RLPlan plan);
 
+/* ---- learned warm start ---------------------------------------------- */
+
+/* A 3-level 3D U-Net (channels 16/32/64, 3x3x3 convolutions, ReLU, skip
+ * connections; bfloat16 weights on MFMA with float32 accumulation) that maps
+ * an observation and its kernel widths to an estimate of the 200-iteration
+ * RL result, used to seed a short RL run (ClarityEx_RLPlanReset).  Volume
+ * dims must be multiples of 4.  Parameters are float32 (He-initialised from
+ * `seed`). */
+typedef struct ClarityEx_UNet_t *ClarityEx_UNet;
+
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetCreate(ClarityEx_UNet *net, unsigned seed);
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetDestroy(ClarityEx_UNet net);
+C_FUNC_DEF int64_t ClarityEx_UNetNumParams(ClarityEx_UNet net);
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetGetParams(ClarityEx_UNet net, float *params);
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetSetParams(ClarityEx_UNet net,
+                                                   const float *params);
+
+/* out = the network's estimate for `obs` (dims floats), in obs units. */
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetPredict(ClarityEx_UNet net, const float *obs,
+                                                 Clarity_Dim3 dims, float sigmaXY,
+                                                 float sigmaZ, float *out,
+                                                 int onDevice);
+
+/* One training step on (obs, target): mean squared error of the estimate
+ * against target, both divided by mean(obs); Adam (beta 0.9 / 0.999) with
+ * `learningRate` (0 = gradients only, no update).  `loss` (may be NULL)
+ * receives the loss, `grads` (host, NumParams floats, may be NULL) the
+ * gradient of every parameter. */
+C_FUNC_DEF ClarityResult_t ClarityEx_UNetTrainStep(
+    ClarityEx_UNet net, const float *obs, const float *target, Clarity_Dim3 dims,
+    float sigmaXY, float sigmaZ, float learningRate, float *loss, float *grads,
+    int onDevice);
+
 /* Blocks until all work queued on the library stream has finished. */
 C_FUNC_DEF ClarityResult_t ClarityEx_Synchronize();
 
