@@ -101,6 +101,16 @@ _SIGS = [
     ("ClarityEx_MaximumLikelihoodDeconvolveBatch", _R,
      [ctypes.c_int, ctypes.POINTER(_F), ctypes.POINTER(Dim3), ctypes.POINTER(_F),
       ctypes.POINTER(Dim3), ctypes.POINTER(_F), ctypes.c_int]),
+    ("ClarityEx_UNetCreate", _R, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]),
+    ("ClarityEx_UNetDestroy", _R, [ctypes.c_void_p]),
+    ("ClarityEx_UNetNumParams", ctypes.c_int64, [ctypes.c_void_p]),
+    ("ClarityEx_UNetGetParams", _R, [ctypes.c_void_p, _F]),
+    ("ClarityEx_UNetSetParams", _R, [ctypes.c_void_p, _F]),
+    ("ClarityEx_UNetPredict", _R,
+     [ctypes.c_void_p, _F, Dim3, ctypes.c_float, ctypes.c_float, _F, ctypes.c_int]),
+    ("ClarityEx_UNetTrainStep", _R,
+     [ctypes.c_void_p, _F, _F, Dim3, ctypes.c_float, ctypes.c_float, ctypes.c_float, _F, _F,
+      ctypes.c_int]),
 ]
 
 EXPORTED = [name for name, _, _ in _SIGS]
@@ -369,3 +379,73 @@ class RLPlan:
 
 def Synchronize():
     _check(lib().ClarityEx_Synchronize(), "ClarityEx_Synchronize")
+
+
+class UNet:
+    """Learned warm-start network (ClarityEx_UNet*).  Parameters are a flat
+    float32 vector: 10 conv weights [27][cout][cin], 10 biases, head (17)."""
+
+    def __init__(self, seed: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().ClarityEx_UNetCreate(ctypes.byref(h), seed), "ClarityEx_UNetCreate")
+        self.h = h
+
+    @property
+    def nparams(self) -> int:
+        return int(lib().ClarityEx_UNetNumParams(self.h))
+
+    def get_params(self):
+        p = np.empty(self.nparams, np.float32)
+        _check(lib().ClarityEx_UNetGetParams(self.h, _p(p)), "ClarityEx_UNetGetParams")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparams:
+            raise ValueError("wrong parameter count")
+        _check(lib().ClarityEx_UNetSetParams(self.h, _p(p)), "ClarityEx_UNetSetParams")
+
+    def predict(self, obs, sigma_xy: float, sigma_z: float):
+        obs = _vol(obs)
+        out = np.empty_like(obs)
+        _check(lib().ClarityEx_UNetPredict(self.h, _p(obs), dim3(obs), sigma_xy, sigma_z,
+                                           _p(out), 0), "ClarityEx_UNetPredict")
+        return out
+
+    def predict_device(self, obs_ptr: int, shape, sigma_xy, sigma_z, out_ptr: int):
+        z, y, x = shape
+        _check(lib().ClarityEx_UNetPredict(self.h, ctypes.cast(obs_ptr, _F), Dim3(x, y, z),
+                                           sigma_xy, sigma_z, ctypes.cast(out_ptr, _F), 1),
+               "ClarityEx_UNetPredict")
+
+    def train_step(self, obs, target, sigma_xy: float, sigma_z: float, lr: float,
+                   want_grads: bool = False):
+        obs = _vol(obs)
+        tgt = _vol(target)
+        loss = ctypes.c_float()
+        g = np.empty(self.nparams, np.float32) if want_grads else None
+        _check(lib().ClarityEx_UNetTrainStep(self.h, _p(obs), _p(tgt), dim3(obs), sigma_xy,
+                                             sigma_z, lr, ctypes.byref(loss),
+                                             _p(g) if g is not None else None, 0),
+               "ClarityEx_UNetTrainStep")
+        return (loss.value, g) if want_grads else loss.value
+
+    def train_step_device(self, obs_ptr: int, tgt_ptr: int, shape, sigma_xy, sigma_z, lr):
+        z, y, x = shape
+        loss = ctypes.c_float()
+        _check(lib().ClarityEx_UNetTrainStep(self.h, ctypes.cast(obs_ptr, _F),
+                                             ctypes.cast(tgt_ptr, _F), Dim3(x, y, z),
+                                             sigma_xy, sigma_z, lr, ctypes.byref(loss),
+                                             None, 1), "ClarityEx_UNetTrainStep")
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().ClarityEx_UNetDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -518,6 +518,7 @@ class UNet {
   long long step_ = 0;
   // activation buffers for the current shape
   Vol v0_{0, 0, 0};
+  bool train_ready_ = false;
   DevBuf in_, act_[10], pool1_, pool2_, cat2_, cat1_, out_, invm_, parts_;
   DevBuf gact_[10], gcat2_, gcat1_, gskip1_, gskip2_, gp1_, gp2_, gd1_, wpart_, bpart_;
   DevBuf lossp_, whp_, sumbuf_;
@@ -584,9 +585,23 @@ class UNet {
   ClarityResult_t shape(Vol v, bool train) {
     if (v.D % 4 || v.H % 4 || v.W % 4 || v.D < 4 || v.H < 4 || v.W < 4)
       return CLARITY_INVALID_ARGUMENT;
-    if (v.D == v0_.D && v.H == v0_.H && v.W == v0_.W && (!train || gd1_.p)) return CLARITY_SUCCESS;
-    v0_ = v;
+    const bool same = v.D == v0_.D && v.H == v0_.H && v.W == v0_.W;
+    if (same && (!train || train_ready_)) return CLARITY_SUCCESS;
     const long long n[3] = {v.n(), v.n() / 8, v.n() / 64};
+    if (!same) {
+      v0_ = Vol{0, 0, 0};  // stays invalid until every buffer is sized
+      train_ready_ = false;
+      CL_TRY(alloc_forward(n));
+      v0_ = v;
+    }
+    if (train && !train_ready_) {
+      CL_TRY(alloc_training(n));
+      train_ready_ = true;
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t alloc_forward(const long long n[3]) {
     CL_TRY(in_.alloc(sizeof(bf16) * n[0] * 16));
     for (int i = 0; i < 10; i++)
       CL_TRY(act_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
@@ -597,7 +612,11 @@ class UNet {
     CL_TRY(out_.alloc(sizeof(float) * n[0]));
     CL_TRY(invm_.alloc(sizeof(float) * 4));
     CL_TRY(parts_.alloc(sizeof(double) * kParts));
-    if (train) {
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t alloc_training(const long long n[3]) {
+    {
       for (int i = 0; i < 10; i++)
         CL_TRY(gact_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
       CL_TRY(gcat2_.alloc(sizeof(bf16) * n[1] * 96));
