@@ -31,6 +31,13 @@ import time
 
 import numpy as np
 
+try:
+    # PyTorch-ROCm ships its own HIP runtime under /opt/rocm's soname; loading
+    # it first keeps one runtime in the process (libclarity binds to it)
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -345,13 +352,17 @@ def main():
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib:
+        # a plain copy kernel (torch's elementwise copy) of known size: reads
+        # and writes exactly n bytes, so counter readings scale against it
+        import torch
         n = args.copy_mib << 20
-        src, dst = hip.malloc(n), hip.malloc(n)
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
         for _ in range(3):
-            hip.d2d(dst, src, n)
-        hip.free(src)
-        hip.free(dst)
-        log(f"[bench] copied {args.copy_mib} MiB device-to-device 3 times")
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"[bench] copied {args.copy_mib} MiB device-to-device 3 times (torch copy kernel)")
     plan.close()
     for p in (d_obs, d_psf, d_out):
         hip.free(p)

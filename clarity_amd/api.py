@@ -117,12 +117,23 @@ EXPORTED = [name for name, _, _ in _SIGS]
 
 
 def lib():
-    """Loads libclarity.so (raises if it has not been built)."""
+    """Loads libclarity.so (raises if it has not been built).
+
+    PyTorch-ROCm wheels ship their own HIP and HSA runtimes under the same
+    sonames as /opt/rocm's (libamdhip64.so.7, libhsa-runtime64.so.1) but
+    load them by file name, so a process that loads libclarity first and
+    torch afterwards ends up with two runtimes and torch finds no GPU.
+    torch, when installed, is therefore imported first: libclarity then
+    binds to the runtime already in the process."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m clarity_amd.build`")
+        try:
+            import torch  # noqa: F401  (one HIP runtime per process)
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in _SIGS:
             f = getattr(L, name)
@@ -326,6 +337,11 @@ class RLPlan:
         else:
             init = _vol(initial)
             _check(lib().ClarityEx_RLPlanReset(self.h, _p(init), 0), "ClarityEx_RLPlanReset")
+
+    def reset_device(self, initial_ptr: int):
+        """Sets the estimate from a device buffer of the image's shape."""
+        _check(lib().ClarityEx_RLPlanReset(self.h, ctypes.cast(initial_ptr, _F), 1),
+               "ClarityEx_RLPlanReset")
 
     def iterate(self, n: int):
         _check(lib().ClarityEx_RLPlanIterate(self.h, n), "ClarityEx_RLPlanIterate")

@@ -1,6 +1,8 @@
 """Builds clarity_amd/libclarity.so from clarity_amd/csrc/*.hip with hipcc.
 
-Every source is compiled for gfx950 only.  Packed FP32 instructions are
+Every source is compiled for gfx950 only.  The raw-image app
+(clarity_amd/apps/DeconvolveRawImages.cpp, host code over the C API) is then
+linked against the library into clarity_amd/bin/.  Packed FP32 instructions are
 disabled (-packed-fp32-ops): the compiler's v_pk_* float forms are kept out
 of every kernel.  Objects go to clarity_amd/_build/, the library next to this
 file, where the ctypes loader finds it.
@@ -19,6 +21,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libclarity.so")
+APPS = os.path.join(HERE, "apps")
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -67,7 +72,26 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
            *[o for o, _ in results]]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_apps()
     return LIB
+
+
+def build_apps() -> list:
+    """Links every apps/*.cpp against libclarity.so (found at run time
+    through the binary's $ORIGIN/.. rpath)."""
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for f in sorted(os.listdir(APPS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, f[:-4])
+        cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+               os.path.join(APPS, f), "-o", exe + ".tmp", "-L", HERE, "-lclarity",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+        subprocess.check_call(cmd)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 if __name__ == "__main__":

@@ -13,7 +13,18 @@ Runtime &rt() {
   return r;
 }
 
-bool ready() { return rt().refcount > 0 && rt().device >= 0; }
+// Entry points called before Clarity_Register initialise the device on
+// first use: the reference's functions work unregistered (src/Clarity.cxx:
+// 20-55 only sets up threads and the device flag), and its own raw-image
+// app never registers (apps/DeconvolveRawImages/DeconvolveRawImages.cxx:
+// 98-100).  The implicit registration counts as one client and holds the
+// device until a Clarity_UnRegister or process exit.
+bool ready() {
+  Runtime &r = rt();
+  std::lock_guard<std::recursive_mutex> lk(r.mu);
+  if (r.refcount > 0 && r.device >= 0) return true;
+  return Clarity_Register() == CLARITY_SUCCESS;
+}
 
 ClarityResult_t from_hip(hipError_t e) {
   if (e == hipSuccess) return CLARITY_SUCCESS;
@@ -119,7 +130,9 @@ ClarityResult_t Clarity_Register() {
   CL_HIP(hipGetDeviceProperties(&prop, dev));
   CL_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));
   r.device = dev;
-  r.device_name = prop.name;
+  // the marketing name can be empty; the architecture name never is
+  r.device_name = prop.name[0] ? std::string(prop.name) + " (" + prop.gcnArchName + ")"
+                               : std::string(prop.gcnArchName);
   r.alloc_count = 0;
   r.fault_every = 0;
   if (const char *f = std::getenv("CLARITY_FAULT_DEVICE_OOM"))

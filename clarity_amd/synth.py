@@ -76,7 +76,7 @@ def np_convolve(img: np.ndarray, kernel: np.ndarray) -> np.ndarray:
     b = np.zeros((wz, wy, wx))
     b[:kz, :ky, :kx] = kernel
     b = np.roll(b, (-(kz // 2), -(ky // 2), -(kx // 2)), axis=(0, 1, 2))
-    r = np.fft.irfftn(np.fft.rfftn(a) * np.fft.rfftn(b), s=(wz, wy, wx))
+    r = np.fft.irfftn(np.fft.rfftn(a) * np.fft.rfftn(b), s=(wz, wy, wx), axes=(0, 1, 2))
     return r[:iz, :iy, :ix]
 
 

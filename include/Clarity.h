@@ -62,7 +62,9 @@ C_FUNC_DEF Clarity_Dim3 Clarity_Dim3FromArray(int dimArray[3]);
 /* Clarity.h.in:137-139 (reference src/Clarity.cxx:46-79).  Reference
  * counted; the first call selects HIP device 0 (or $CLARITY_DEVICE) and
  * creates the library stream.  Returns CLARITY_INVALID_OPERATION when no HIP
- * device is usable. */
+ * device is usable.  As with the reference, calling it is optional: an entry
+ * point used before any Register registers implicitly (one client, held until
+ * an UnRegister or process exit). */
 C_FUNC_DEF ClarityResult_t Clarity_Register();
 
 /* Clarity.h.in:147-149 (reference src/Clarity.cxx:82-91).  The last matching

@@ -326,3 +326,27 @@ def test_register_refcount(clarity_gpu):
     # still registered by the session fixture
     a = _rand((4, 4, 4), 13)
     assert api.FFT_R2C(a).shape == (4, 4, 3)
+
+
+def test_calls_without_register(tmp_path):
+    """A process that never calls Clarity_Register still gets results (the
+    reference's functions work unregistered; its raw-image app relies on
+    it): the first entry point registers implicitly."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "import oracle\n"
+        "from clarity_amd import api, synth\n"
+        "c = synth.make_config(1)\n"
+        "out = api.MaximumLikelihoodDeconvolve(c.obs, c.psf, 3)\n"
+        "ref = oracle.ml(c.obs, c.psf, 3)\n"
+        "e = np.linalg.norm(out.astype(np.float64) - ref) / np.linalg.norm(ref)\n"
+        "print('rel', e)\n"
+        "sys.exit(0 if e <= 1e-4 else 3)\n" % root)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0
