@@ -7,9 +7,10 @@
 // float32 masters with bfloat16 copies in two layouts, [tap][co][ci] for the
 // forward pass and [tap][ci][co] for the input gradient.  Every 3x3x3
 // convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16 (float32
-// accumulation): M = voxels, N = output channels, K = 27 taps x input
-// channels in chunks of 16.  The weight gradient is the same GEMM with K =
-// voxels, staged through LDS, reduced over voxel chunks in a fixed order.
+// accumulation) over K = 27 taps x input channels in chunks of 16; the
+// forward pass stages input tiles in LDS.  The weight gradient is the same
+// GEMM with K = voxels, staged through LDS, reduced over voxel chunks in a
+// fixed order.
 //
 // Network (C = channels):
 //   in(16: obs/mean, sigma_xy/4, sigma_z/8, zeros) -> c1a(16) -> e1(16)
@@ -37,6 +38,8 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return (bf16)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+// max(r, 0) with NaN -> +0 and -0 -> +0 (a stored activation is never -0)
+__device__ __forceinline__ float relu(float r) { return r > 0.f ? r : 0.f; }
 
 struct Vol {
   int D, H, W;
@@ -51,53 +54,106 @@ __device__ __forceinline__ void tap_offset(int tap, int &dz, int &dy, int &dx) {
 
 // ------------------------------------------------------------------ conv
 
-// Y = relu(conv(X, W) + b).  Block: 4 wavefronts x 16 voxels along x.
-// A fragment (voxel m = lane%16, input channels c0 + 4*(lane/16) + j),
-// B fragment (output channel n = lane%16, same channels), D fragment
-// (voxel 4*(lane/16) + j, output channel lane%16).
-template <int CIN, int COUT>
+// Y = relu(conv(X, W) + b).  A block computes a 64 (x) by 4 (y) tile of
+// output voxels of one z plane, one wavefront per row, all COUT channels.
+// Input channels go in chunks of 16: the chunk's (3 x 6 x 66)-voxel input
+// tile is staged in LDS as four planes of channel quads ([quad][voxel], 8 B
+// each, so 16 consecutive voxels of one quad are 128 contiguous bytes), and
+// every tap reads its fragments from there.  The MFMA computes the
+// transposed product: operand A = weights (row = output channel,
+// k = input channel), operand B = input (column = voxel), D row = output
+// channel 4*(lane/16) + j, column = voxel lane%16, so each lane stores four
+// consecutive output channels of one voxel (8 B).
+//
+// With UP the input is the channel concatenation up(Xc) | S of a coarse
+// tensor Xc (CA channels, nearest-neighbour 2x upsampled on the fly) and a
+// fine tensor S (CIN - CA channels); otherwise X (CIN channels).
+constexpr int kXT = 64, kYT = 4;                                // output tile
+constexpr int kTX = kXT + 2, kTY = kYT + 2, kTZ = 3;             // input tile
+constexpr int kTV = kTX * kTY * kTZ;                            // 1188 voxels
+
+template <int CIN, int COUT, int CA, bool UP>
 __global__ void __launch_bounds__(256)
-    k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ Wf,
-               const float *__restrict__ bias, bf16 *__restrict__ Y, Vol v) {
+    k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
+               bf16 *__restrict__ Y, Vol v) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
+                "channel chunks of 16");
+  __shared__ uint2 tile[4][kTV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const int x0 = blockIdx.x * 64 + wave * 16;
-  const int y = blockIdx.y, z = blockIdx.z;
-  const int xv = x0 + m;
-  f32x4 acc[COUT / 16];
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z = blockIdx.z;
+  f32x4 acc[4][COUT / 16];
 #pragma unroll
-  for (int c = 0; c < COUT / 16; c++) acc[c] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 27; tap++) {
-    int dz, dy, dx;
-    tap_offset(tap, dz, dy, dx);
-    const int zz = z + dz, yy = y + dy, xx = xv + dx;
-    const bool in = xv < v.W && zz >= 0 && zz < v.D && yy >= 0 && yy < v.H && xx >= 0 &&
-                    xx < v.W;
-    const bf16 *xp = X + (((long long)zz * v.H + yy) * v.W + xx) * CIN + 4 * kq;
-    const bf16 *wp = Wf + ((long long)tap * COUT + m) * CIN + 4 * kq;
+  for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-    for (int c0 = 0; c0 < CIN; c0 += 16) {
-      s16x4 a = {0, 0, 0, 0};
-      if (in) a = *reinterpret_cast<const s16x4 *>(xp + c0);
+    for (int ct = 0; ct < COUT / 16; ct++) acc[mt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < CIN; c0 += 16) {
+    __syncthreads();  // every wavefront is done with the previous chunk
+    // stage: two 16-byte loads (8 channels each) per tile voxel
+    for (int i = threadIdx.x; i < 2 * kTV; i += 256) {
+      const int vox = i >> 1, half = i & 1;
+      const int tx = vox % kTX, t2 = vox / kTX;
+      const int ty = t2 % kTY, tz = t2 / kTY;
+      const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z + tz - 1;
+      uint4 val = {0u, 0u, 0u, 0u};
+      if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H && gz >= 0 && gz < v.D) {
+        const int c = c0 + 8 * half;
+        const bf16 *p;
+        if (!UP) {
+          p = X + (((long long)gz * v.H + gy) * v.W + gx) * CIN + c;
+        } else if (c < CA) {
+          const int H2 = v.H >> 1, W2 = v.W >> 1;
+          p = X + (((long long)(gz >> 1) * H2 + (gy >> 1)) * W2 + (gx >> 1)) * CA + c;
+        } else {
+          p = S + (((long long)gz * v.H + gy) * v.W + gx) * (CIN - CA) + (c - CA);
+        }
+        val = *reinterpret_cast<const uint4 *>(p);
+      }
+      tile[2 * half][vox] = make_uint2(val.x, val.y);
+      tile[2 * half + 1][vox] = make_uint2(val.z, val.w);
+    }
+    __syncthreads();
+    for (int tap = 0; tap < 27; tap++) {
+      int dz, dy, dx;
+      tap_offset(tap, dz, dy, dx);
+      s16x4 w[COUT / 16];
 #pragma unroll
-      for (int ct = 0; ct < COUT / 16; ct++) {
-        const s16x4 b = *reinterpret_cast<const s16x4 *>(wp + ct * 16 * CIN + c0);
-        acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+      for (int ct = 0; ct < COUT / 16; ct++)
+        w[ct] = *reinterpret_cast<const s16x4 *>(
+            Wf + ((long long)tap * COUT + ct * 16 + m) * CIN + c0 + 4 * kq);
+      const int rb = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const uint2 t = tile[kq][rb + mt * 16 + m];
+        s16x4 a;
+        a[0] = (short)(t.x & 0xffffu);
+        a[1] = (short)(t.x >> 16);
+        a[2] = (short)(t.y & 0xffffu);
+        a[3] = (short)(t.y >> 16);
+#pragma unroll
+        for (int ct = 0; ct < COUT / 16; ct++)
+          acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w[ct], a, acc[mt][ct], 0, 0, 0);
       }
     }
   }
+  const int y = y0 + wave;
+  if (y >= v.H) return;
   const long long rowbase = ((long long)z * v.H + y) * v.W;
 #pragma unroll
   for (int ct = 0; ct < COUT / 16; ct++) {
-    const int co = ct * 16 + m;
-    const float bb = bias[co];
+    const int co = ct * 16 + 4 * kq;
+    const float4 bb = *reinterpret_cast<const float4 *>(bias + co);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int xo = x0 + 4 * kq + j;
+    for (int mt = 0; mt < 4; mt++) {
+      const int xo = x0 + mt * 16 + m;
       if (xo < v.W) {
-        float r = acc[ct][j] + bb;
-        r = r > 0.f ? r : 0.f;
-        Y[(rowbase + xo) * COUT + co] = f2bf(r);
+        const float r0 = relu(acc[mt][ct][0] + bb.x), r1 = relu(acc[mt][ct][1] + bb.y);
+        const float r2 = relu(acc[mt][ct][2] + bb.z), r3 = relu(acc[mt][ct][3] + bb.w);
+        uint2 o;
+        o.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
+        o.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+        *reinterpret_cast<uint2 *>(Y + (rowbase + xo) * COUT + co) = o;
       }
     }
   }
@@ -477,11 +533,12 @@ constexpr ConvSpec kConv[10] = {{16, 16}, {16, 16}, {16, 32}, {32, 32}, {32, 64}
                                 {64, 64}, {96, 32}, {32, 32}, {48, 16}, {16, 16}};
 constexpr int kLevel[10] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
 
-template <int CIN, int COUT>
-void conv_fwd_t(const bf16 *X, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
+template <int CIN, int COUT, int CA = 0, bool UP = false>
+void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
                 hipStream_t s) {
-  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT>), dim3((v.W + 63) / 64, v.H, v.D), dim3(256), 0,
-                     s, X, Wf, b, Y, v);
+  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
+                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
+                     X, S, Wf, b, Y, v);
 }
 template <int CIN, int COUT>
 void conv_bwd_t(const bf16 *G, const bf16 *Yo, const bf16 *X, const bf16 *Wb, bf16 *GX,
@@ -607,8 +664,6 @@ class UNet {
       CL_TRY(act_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
     CL_TRY(pool1_.alloc(sizeof(bf16) * n[1] * 16));
     CL_TRY(pool2_.alloc(sizeof(bf16) * n[2] * 32));
-    CL_TRY(cat2_.alloc(sizeof(bf16) * n[1] * 96));
-    CL_TRY(cat1_.alloc(sizeof(bf16) * n[0] * 48));
     CL_TRY(out_.alloc(sizeof(float) * n[0]));
     CL_TRY(invm_.alloc(sizeof(float) * 4));
     CL_TRY(parts_.alloc(sizeof(double) * kParts));
@@ -617,6 +672,8 @@ class UNet {
 
   ClarityResult_t alloc_training(const long long n[3]) {
     {
+      CL_TRY(cat2_.alloc(sizeof(bf16) * n[1] * 96));
+      CL_TRY(cat1_.alloc(sizeof(bf16) * n[0] * 48));
       for (int i = 0; i < 10; i++)
         CL_TRY(gact_[i].alloc(sizeof(bf16) * n[kLevel[i]] * kConv[i].cout));
       CL_TRY(gcat2_.alloc(sizeof(bf16) * n[1] * 96));
@@ -636,8 +693,10 @@ class UNet {
     return CLARITY_SUCCESS;
   }
 
-  // forward on a device observation; leaves activations for backward
-  ClarityResult_t forward(const float *obs_d, float sxy, float sz, hipStream_t s) {
+  // forward on a device observation; leaves activations for backward (and,
+  // with keep_concat, the decoder's concatenated inputs)
+  ClarityResult_t forward(const float *obs_d, float sxy, float sz, bool keep_concat,
+                          hipStream_t s) {
     const Vol v = v0_;
     const Vol v1{v.D / 2, v.H / 2, v.W / 2}, v2{v.D / 4, v.H / 4, v.W / 4};
     const Vol lv[3] = {v, v1, v2};
@@ -665,6 +724,20 @@ class UNet {
       if (i == 4)
         hipLaunchKernelGGL(k_pool, dim3(grid_for(v2.n() * 32)), dim3(256), 0, s,
                            (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1, 32);
+      bf16 *Y = (bf16 *)act_[i].p;
+      const float *b = p_.f() + off_[10 + i];
+      const bf16 *Wf = (bf16 *)wf_[i].p;
+      const Vol vv = lv[kLevel[i]];
+      // decoder inputs up(coarse) | skip: read in place for inference; the
+      // training pass materialises them (the weight gradient reads them)
+      if (i == 6 && !keep_concat) {
+        conv_fwd_t<96, 32, 64, true>((bf16 *)act_[5].p, (bf16 *)act_[3].p, Wf, b, Y, v1, s);
+        continue;
+      }
+      if (i == 8 && !keep_concat) {
+        conv_fwd_t<48, 16, 32, true>((bf16 *)act_[7].p, (bf16 *)act_[1].p, Wf, b, Y, v, s);
+        continue;
+      }
       if (i == 6)
         hipLaunchKernelGGL(k_up_concat, dim3(grid_for(v1.n() * 96)), dim3(256), 0, s,
                            (bf16 *)act_[5].p, (bf16 *)act_[3].p, (bf16 *)cat2_.p, v1, 64, 96);
@@ -672,11 +745,7 @@ class UNet {
         hipLaunchKernelGGL(k_up_concat, dim3(grid_for(v.n() * 48)), dim3(256), 0, s,
                            (bf16 *)act_[7].p, (bf16 *)act_[1].p, (bf16 *)cat1_.p, v, 32, 48);
       const bf16 *X = src[i];
-      bf16 *Y = (bf16 *)act_[i].p;
-      const float *b = p_.f() + off_[10 + i];
-      const bf16 *Wf = (bf16 *)wf_[i].p;
-      const Vol vv = lv[kLevel[i]];
-      CONV_SWITCH(i, (conv_fwd_t<A, B>(X, Wf, b, Y, vv, s)));
+      CONV_SWITCH(i, (conv_fwd_t<A, B>(X, nullptr, Wf, b, Y, vv, s)));
     }
     CL_TRY(launch_check());
     return CLARITY_SUCCESS;
@@ -800,7 +869,7 @@ ClarityResult_t unet_run(ClarityEx_UNet h, const float *obs, const float *tgt,
       tgt_d = b.f();
     }
   }
-  CL_TRY(n.forward(obs_d, sxy, sz, s));
+  CL_TRY(n.forward(obs_d, sxy, sz, train, s));
   if (train) {
     CL_TRY(n.head(obs_d, n.out_.f(), false, s));
     CL_TRY(n.backward(tgt_d, lr, loss, s));

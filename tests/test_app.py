@@ -80,16 +80,17 @@ def _compare(got, ref_f):
     ref = cast_like_reference(ref_f)
     diff = (got.astype(np.int64) - ref.astype(np.int64)) % 65536
     diff = np.minimum(diff, 65536 - diff)      # distance on the 16-bit circle
-    # float results agree to ~1e-6 relative; a truncation can land on either
-    # side of an integer only where the value is within that of one
+    # float results agree to ~1e-6 relative (0.05 at 5e4); a truncation can
+    # land on either side of an integer only where the value is that close
+    # to one
     assert diff.max() <= 1
-    assert np.count_nonzero(diff) <= max(3, ref.size // 1000)
+    assert np.count_nonzero(diff) <= max(3, ref.size // 20)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scale", [1.0, 120.0])
+@pytest.mark.parametrize("scale", [1.0, 1500.0])
 def test_app_matches_oracle(tmp_path, scale):
-    """Scale 120 pushes deconvolved bead peaks past 65535, where the
+    """Scale 1500 pushes deconvolved bead peaks past 65535, where the
     reference's unclamped cast wraps."""
     img, psf, fi, fp = _write_inputs(tmp_path, scale)
     z, y, x = img.shape
@@ -109,4 +110,4 @@ def test_app_matches_oracle(tmp_path, scale):
     ref3 = oracle.ml(img.astype(np.float32), psf.astype(np.float32), 3)
     want = np.clip(np.trunc(ref3), 0, 65535).astype(np.int64)
     d = np.abs(got.astype(np.int64) - want)
-    assert d.max() <= 1 and np.count_nonzero(d) <= max(3, want.size // 1000)
+    assert d.max() <= 1 and np.count_nonzero(d) <= max(3, want.size // 20)
