@@ -17,6 +17,11 @@ Also reported:
   cpu_baseline  the CPU oracle (the reference's CPU algorithm restated in C,
                 FFTW3-API FFTs via MKL when present, plan per FFT, float32
                 serial energy sum), timed on this host on a bounded sample.
+  learned_warm_start
+                the metric's second half: the committed network + 20 RL
+                iterations against plain RL on the same volume, judged by
+                plain RL after 200 iterations (relative L2, I-divergence),
+                with plain RL's time to reach the same error.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -221,6 +226,8 @@ def main():
     ap.add_argument("--shape", type=str, default="128,512,512", help="image z,y,x")
     ap.add_argument("--cpu-iterations", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned warm start's time-to-accuracy measurement")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device 3 times "
                          "(a plain copy kernel of known size, to calibrate rocprofv3 "
@@ -299,7 +306,7 @@ def main():
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tf):
-            traffic = json.load(open(tf)).get(name)
+            traffic = json.load(open(tf)).get("bytes_per_launch", {}).get(name)
         roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": byts,
@@ -328,8 +335,36 @@ def main():
                          f"{total:.2f} s call; FFT {backend}, serial float32 sum"}
         log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s")
 
+    # time-to-accuracy of the learned warm start against plain RL, on the
+    # same volume (network + 20 RL iterations vs plain RL, judged against
+    # plain RL after 200 iterations)
+    learned = None
+    if not args.no_learned and shape == (128, 512, 512):
+        from clarity_amd import learned as L
+        if os.path.exists(L.WEIGHTS):
+            log("[bench] learned warm start: network + 20 RL vs plain RL-200")
+            params, card = L.load()
+            net = api.UNet()
+            net.set_params(params)
+            ev = L.evaluate(net, case=type("Case", (), {"obs": obs, "psf": psf})())
+            net.close()
+            learned = {
+                "learned_ms": round(ev["seconds_learned"] * 1e3, 2),
+                "network_ms": round(ev["seconds_network"] * 1e3, 2),
+                "plain_rl200_ms": round(ev["seconds_plain"] * 1e3, 2),
+                "learned_rel_l2_vs_rl200": round(ev["learned_rel_l2"], 4),
+                "learned_idiv_rel_gap": round(ev["learned_idiv_rel_gap"], 5),
+                "plain_iterations_to_same_error": None if ev["plain_iterations_to_same_error"]
+                is None else round(ev["plain_iterations_to_same_error"], 1),
+                "time_to_accuracy_speedup": None if ev["time_to_accuracy_speedup"] is None
+                else round(ev["time_to_accuracy_speedup"], 2),
+                "wall_ratio_vs_rl200": round(ev["wall_ratio_vs_judge"], 2),
+                "accepted_at_1e-3": ev["accepted"],
+            }
+            log(f"[bench] learned {learned}")
+
     line = {
-        "metric": "Mvoxel-iters/s RL 512x512x128 fp32",
+        "metric": "Mvoxel-iters/s RL 512x512x128 fp32; time-to-accuracy learned vs plain RL",
         "value": round(value, 2),
         "unit": "Mvoxel-iters/s",
         "n_gpus": 1,
@@ -346,14 +381,18 @@ def main():
                                f"work {nx}x{ny}x{nz}",
                    "engine": plan.engine, "iterations": args.iterations,
                    "ms_per_iteration": round(1e3 * it_s, 4),
-                   "dense_model_GBps": round((4 * R + 18 * C) / it_s / 1e9, 1)},
+                   "dense_model_GBps": round((4 * R + 18 * C) / it_s / 1e9, 1),
+                   "pruned_model_GBps": round((0.5 * R + 8 * C) / it_s / 1e9, 1),
+                   "pruned_model_frac": round((0.5 * R + 8 * C) / it_s / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_warm_start": learned,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib:
-        # a plain copy kernel (torch's elementwise copy) of known size: reads
-        # and writes exactly n bytes, so counter readings scale against it
+        # a plain copy kernel of known size (torch's device copy, which runs
+        # the HIP runtime's copyBuffer kernel): reads and writes exactly n
+        # bytes, so counter readings scale against it
         import torch
         n = args.copy_mib << 20
         src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
