@@ -101,6 +101,12 @@ struct Dims {
 inline Dims dims_of(Clarity_Dim3 d) { return Dims{d.x, d.y, d.z}; }
 inline bool dims_ok(Clarity_Dim3 d) { return d.x > 0 && d.y > 0 && d.z > 0; }
 
+// The reference's divide guard compares a float |d| with the double 1e-5
+// (src/ComputePrimitives.cxx:143-148).  No float lies in [1e-5, 0x1.4f8b5ap-17)
+// (0x1.4f8b58p-17 < 1e-5 < 0x1.4f8b5ap-17 are neighbours), so the same test
+// in float is |d| < kDivGuard.
+constexpr float kDivGuard = 0x1.4f8b5ap-17f;
+
 // 1/(nz*ny*nx) as the reference computes it: ((float)nz * ny) * nx in float
 // (reference src/Convolve.cxx:199).
 inline float inv_volume(int nx, int ny, int nz) {

@@ -18,7 +18,10 @@
 //   finish  partials                  -> c' = E_obs / sum(u')
 //
 // with the estimate kept as u and its energy scale c in HBM (g = c*u), so no
-// iteration waits on the host.  The spectrum S has layout (z, kx, y), y
+// iteration waits on the host.  Between consecutive iterations x_last and
+// the next x_fwd run as one kernel (x_last_fwd: u' is transformed while
+// still in registers), so a run of n iterations launches x_fwd and x_last
+// once and x_last_fwd n - 1 times.  The spectrum S has layout (z, kx, y), y
 // fastest, z < iz only; the OTF has layout (kz, kx, ky).  Every transform is
 // a register-resident Stockham FFT: a thread holds E elements of its pencil
 // (element t + T*e in v[e]), butterflies of radix E run in registers and
@@ -166,13 +169,33 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Row q of an x-pass tile, XOR-swizzled inside 16-element blocks so that a
-// column access (same pos, 16 consecutive rows) hits distinct banks.  A row
-// is transformed by T <= 64 consecutive lanes, i.e. inside one wavefront.
+// LDS slot of element pos of an FFT line stored with one pad slot after
+// every 2^PS elements.  A Stockham stage of radix R <= 2^PS writes its
+// outputs at stride R; unpadded, the lanes of such a write pile onto a few
+// banks, padded they spread over all of them.
+template <int PS>
+__device__ __forceinline__ int padded(int pos) {
+  return pos + (pos >> PS);
+}
+template <int N, int PS>
+constexpr int kPadded = N + (N >> PS);
+// pad interval for E elements per thread (the first stage's radix)
+template <int E>
+constexpr int kPadShift = (E == 16) ? 4 : 3;
+
+// Row q of an x-pass tile: padded, and rows kRowStride apart, an odd number
+// of slots, so that a column access (same pos, consecutive rows) also hits
+// distinct banks.  Positions stay affine in the element index, so a lane
+// addresses all its elements from one register plus immediate offsets.  A
+// row is transformed by T <= 64 consecutive lanes, i.e. inside one
+// wavefront.
+template <int M, int PS>
+constexpr int kRowStride = kPadded<M, PS> + 1;
+
+template <int PS>
 struct RowLds {
   float2 *base;
-  int sw;
-  __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos ^ sw]; }
+  __device__ __forceinline__ float2 &operator()(int pos) const { return base[padded<PS>(pos)]; }
   __device__ __forceinline__ void sync() const { wave_sync(); }
 };
 // Pencil q of an interleaved tile: element pos at pos*TP + q.  A pencil's
@@ -183,11 +206,12 @@ struct InterLds {
   __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos * TP]; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
 };
-// A contiguous pencil transformed by T consecutive lanes; WAVE when T <= 64.
-template <bool WAVE>
+// A contiguous (padded) pencil transformed by T consecutive lanes; WAVE when
+// T <= 64.
+template <bool WAVE, int PS>
 struct PencilLds {
   float2 *base;
-  __device__ __forceinline__ float2 &operator()(int pos) const { return base[pos]; }
+  __device__ __forceinline__ float2 &operator()(int pos) const { return base[padded<PS>(pos)]; }
   __device__ __forceinline__ void sync() const {
     if constexpr (WAVE)
       wave_sync();
@@ -296,19 +320,20 @@ __device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y)
 }
 
 // Writes X[0..M] of the TP rows held (as Z = FFT_M(x_even + i x_odd)) in
-// LDS, to S with consecutive rows on consecutive lanes.
-template <int M, int TP, int NT>
+// LDS (RowLds<PS> layout, rows kRowStride<M, PS> apart), to S with consecutive
+// rows on consecutive lanes.
+template <int M, int TP, int NT, int PS>
 __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restrict__ S,
                                              const float2 *twN, const Geom &g, int z,
                                              int y0) {
+  constexpr int MP = kRowStride<M, PS>;
   for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
     const int qq = idx % TP;
     const int k = idx / TP;
-    const int sw = qq & 15;
     const int kk = (k == M) ? 0 : k;  // Z is M-periodic
     const int km = (k == 0) ? 0 : M - k;
-    const float2 zk = lds[qq * M + (kk ^ sw)];
-    const float2 zm = lds[qq * M + (km ^ sw)];
+    const float2 zk = lds[qq * MP + padded<PS>(kk)];
+    const float2 zm = lds[qq * MP + padded<PS>(km)];
     const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);  // Zk + conj(Zm)
     const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Zk - conj(Zm)
     const float2 wd = cmul(twN[k], d);                        // w^k d
@@ -321,28 +346,29 @@ __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restri
 // Loads X[0..M] of TP rows from S into LDS (row layout); the real parts of
 // X[0] and X[M] share slot 0 (their imaginary parts are ignored, as a
 // multi-dimensional C2R ignores them).
-template <int M, int TP, int NT>
+template <int M, int TP, int NT, int PS>
 __device__ __forceinline__ void load_spectrum(float2 *lds, const float2 *__restrict__ S,
                                               const Geom &g, int z, int y0) {
+  constexpr int MP = kRowStride<M, PS>;
   for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
     const int qq = idx % TP;
     const int k = idx / TP;
     float2 X = make_float2(0.f, 0.f);
     if (y0 + qq < g.iy) X = S[s_index(g, z, k, y0 + qq)];
-    const int sw = qq & 15;
+    float2 *row = lds + qq * MP;
     if (k == 0)
-      lds[qq * M + sw].x = X.x;
+      row[0].x = X.x;
     else if (k == M)
-      lds[qq * M + sw].y = X.x;
+      row[0].y = X.x;
     else
-      lds[qq * M + (k ^ sw)] = X;
+      row[padded<PS>(k)] = X;
   }
 }
 
 // Forms Zc[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]) for the
 // thread's k = t + T*e; its inverse FFT is x[2m] + i x[2m+1].
-template <int M, int E>
-__device__ __forceinline__ void pack_inverse(float2 (&v)[E], const RowLds &L, int t,
+template <int M, int E, class L_t>
+__device__ __forceinline__ void pack_inverse(float2 (&v)[E], const L_t &L, int t,
                                              const float2 *twN) {
   constexpr int T = M / E;
 #pragma unroll
@@ -399,9 +425,9 @@ __device__ __forceinline__ int x_tile(int ntiles) {
   }
 }
 
-template <int M, int TP>
+template <int M, int TP, int PS>
 struct XSmem {
-  float2 tile[TP * M];
+  float2 tile[TP * kRowStride<M, PS>];
   float2 twM[M];
   float2 twN[M + 1];
 };
@@ -412,7 +438,8 @@ __global__ void __launch_bounds__(TP *(M / E))
             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
-  __shared__ XSmem<M, TP> sm;
+  constexpr int PS = kPadShift<E>;
+  __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
   const int tile = x_tile<TP>(g.iz * ntile);
@@ -425,12 +452,12 @@ __global__ void __launch_bounds__(TP *(M / E))
   stage_table<NT>(sm.twM, gtwM, M);
   stage_table<NT>(sm.twN, gtwN, M + 1);
   __syncthreads();
-  const RowLds L{sm.tile + q * M, q & 15};
+  const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
-  unpack_store<M, TP, NT>(sm.tile, S, sm.twN, g, z, y0);
+  unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
 template <int M, int E, int TP>
@@ -440,7 +467,8 @@ __global__ void __launch_bounds__(TP *(M / E))
             const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
-  __shared__ XSmem<M, TP> sm;
+  constexpr int PS = kPadShift<E>;
+  __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
   const int tile = x_tile<TP>(g.iz * ntile);
@@ -455,15 +483,16 @@ __global__ void __launch_bounds__(TP *(M / E))
   load_row_pairs<E, T>(o, obs + ((long long)z * g.iy + y) * g.ix, valid, t, g.ix);
   stage_table<NT>(sm.twM, gtwM, M);
   stage_table<NT>(sm.twN, gtwN, M + 1);
-  load_spectrum<M, TP, NT>(sm.tile, S, g, z, y0);
+  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
   __syncthreads();
-  const RowLds L{sm.tile + q * M, q & 15};
+  const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
   // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
-  // |s1| < 1e-5 (compared in double) gives 0, as the reference's ML step
+  // |s1| < 1e-5 (the reference compares in double; kDivGuard is the same
+  // test in float) gives 0, as the reference's ML step
   // (src/ComputePrimitives.cxx:143-148, src/MaximumLikelihoodDeconvolve.cxx:62)
   const float c = cptr[0];
 #pragma unroll
@@ -472,11 +501,11 @@ __global__ void __launch_bounds__(TP *(M / E))
     float r0 = 0.f, r1 = 0.f;
     if (valid && n0 < g.ix) {
       const float s = c * v[e].x;
-      r0 = ((double)fabsf(s) < 1e-5) ? 0.f : o[e].x / s;
+      r0 = (fabsf(s) < kDivGuard) ? 0.f : o[e].x / s;
     }
     if (valid && n0 + 1 < g.ix) {
       const float s = c * v[e].y;
-      r1 = ((double)fabsf(s) < 1e-5) ? 0.f : o[e].y / s;
+      r1 = (fabsf(s) < kDivGuard) ? 0.f : o[e].y / s;
     }
     v[e] = make_float2(r0, r1);
   }
@@ -484,7 +513,7 @@ __global__ void __launch_bounds__(TP *(M / E))
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
-  unpack_store<M, TP, NT>(sm.tile, S, sm.twN, g, z, y0);
+  unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
 template <int M, int E, int TP>
@@ -494,7 +523,8 @@ __global__ void __launch_bounds__(TP *(M / E))
              const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
-  __shared__ XSmem<M, TP> sm;
+  constexpr int PS = kPadShift<E>;
+  __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
   const int tile = x_tile<TP>(g.iz * ntile);
@@ -508,9 +538,9 @@ __global__ void __launch_bounds__(TP *(M / E))
   load_row_pairs<E, T>(uo, urow, valid, t, g.ix);
   stage_table<NT>(sm.twM, gtwM, M);
   stage_table<NT>(sm.twN, gtwN, M + 1);
-  load_spectrum<M, TP, NT>(sm.tile, S, g, z, y0);
+  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
   __syncthreads();
-  const RowLds L{sm.tile + q * M, q & 15};
+  const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
@@ -544,6 +574,75 @@ __global__ void __launch_bounds__(TP *(M / E))
   if (threadIdx.x == 0) partials[tile] = sh[0];
 }
 
+// x_last of one iteration fused with x_fwd of the next: the updated rows
+// u' = (c*u)*s2 are written, and while they are still in registers their R2C
+// along x (the next iteration's first pass, which needs no scale: c enters
+// at x_mid) goes to S.  The tile's spectrum rows are read completely before
+// any of them is rewritten, and no other workgroup touches them.
+template <int M, int E, int TP>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
+                 const float *__restrict__ cptr, double *__restrict__ partials,
+                 const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  constexpr int T = M / E;
+  constexpr int NT = TP * T;
+  constexpr int PS = kPadShift<E>;
+  __shared__ XSmem<M, TP, PS> sm;
+  const int q = threadIdx.x / T, t = threadIdx.x % T;
+  const int ntile = (g.iy + TP - 1) / TP;
+  const int tile = x_tile<TP>(g.iz * ntile);
+  if (tile < 0) return;  // whole workgroup: padding block of the remap
+  const int z = tile / ntile;
+  const int y0 = (tile % ntile) * TP;
+  const int y = y0 + q;
+  const bool valid = y < g.iy;
+  float *urow = u + ((long long)z * g.iy + y) * g.ix;
+  float2 uo[E];
+  load_row_pairs<E, T>(uo, urow, valid, t, g.ix);
+  stage_table<NT>(sm.twM, gtwM, M);
+  stage_table<NT>(sm.twN, gtwN, M + 1);
+  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
+  __syncthreads();
+  const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
+  float2 v[E];
+  pack_inverse<M, E>(v, L, t, sm.twN);
+  L.sync();  // the row's own lanes read it; the FFT overwrites it next
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  // the same update and sum as k_x_last
+  const float c = cptr[0];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int n0 = 2 * (t + T * e);
+    float u0 = 0.f, u1 = 0.f;
+    if (valid && n0 < g.ix) {
+      u0 = (c * uo[e].x) * v[e].x;
+      urow[n0] = u0;
+      acc += (double)u0;
+    }
+    if (valid && n0 + 1 < g.ix) {
+      u1 = (c * uo[e].y) * v[e].y;
+      urow[n0 + 1] = u1;
+      acc += (double)u1;
+    }
+    v[e] = make_float2(u0, u1);
+  }
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+#pragma unroll
+  for (int e = 0; e < E; e++) L(t + T * e) = v[e];
+  __syncthreads();
+  unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
+  double *sh = reinterpret_cast<double *>(sm.tile);
+  __syncthreads();
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[tile] = sh[0];
+}
+
 // ------------------------------------------------------------- y and z
 
 template <int N, int E, int PPW, int SIGN>
@@ -552,7 +651,11 @@ __global__ void __launch_bounds__(PPW *(N / E))
         int Lin, int Lout) {
   constexpr int T = N / E;
   constexpr int NT = PPW * T;
-  __shared__ float2 lds[PPW * N];
+  // unpadded (PS beyond any position): measured faster at 1024 points, where
+  // the padding costs a workgroup of occupancy per CU
+  constexpr int PS = 30;
+  constexpr int NP = kPadded<N, PS>;
+  __shared__ float2 lds[PPW * NP];
   __shared__ float2 tw[N];
   const int pl = threadIdx.x / T, t = threadIdx.x % T;
   const long long p = (long long)blockIdx.x * PPW + pl;
@@ -566,7 +669,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
   stage_table<NT>(tw, gtw, N);
   __syncthreads();
-  const PencilLds<(T <= 64)> L{lds + pl * N};
+  const PencilLds<(T <= 64), PS> L{lds + pl * NP};
   fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename Plan<N, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -575,9 +678,9 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
 }
 
-// PRE: the OTF column is loaded into registers at the start, overlapping the
-// forward transform (more registers, more loads in flight).
-template <int N, int E, int TP, bool PRE>
+// A workgroup transforms TP adjacent ky columns of one kx (TP * 8 bytes
+// contiguous per z plane); TP lanes of a wavefront read one plane.
+template <int N, int E, int TP>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
         const float2 *__restrict__ gtw, Geom g, float scale) {
@@ -590,16 +693,16 @@ __global__ void __launch_bounds__(TP *(N / E))
   const int kx = blockIdx.x / ntile;
   const int y0 = (blockIdx.x % ntile) * TP;
   const long long col = (long long)kx * g.ny + y0 + q;
+  // element (t + T*e) of the column: one 64-bit step of T planes per e
+  const long long step = (long long)T * g.sz;
+  float2 *const s0 = S + (long long)t * g.sz + col;
+  const float2 *const h0 = otf + (long long)t * g.sz + col;
   float2 v[E];
+  {
+    const float2 *p = s0;
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int z = t + T * e;
-    v[e] = (z < g.iz) ? S[z * g.sz + col] : make_float2(0.f, 0.f);
-  }
-  float2 h[PRE ? E : 1];
-  if constexpr (PRE) {
-#pragma unroll
-    for (int e = 0; e < E; e++) h[e] = otf[(t + T * e) * g.sz + col];
+    for (int e = 0; e < E; e++, p += step)
+      v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
   }
   stage_table<NT>(tw, gtw, N);
   __syncthreads();
@@ -608,38 +711,36 @@ __global__ void __launch_bounds__(TP *(N / E))
   // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    float2 hv;
-    if constexpr (PRE)
-      hv = h[e];
-    else
-      hv = otf[(t + T * e) * g.sz + col];
+    const float2 hv = h0[e * step];
     const float2 a = v[e];
     v[e] = make_float2(scale * (a.x * hv.x - a.y * hv.y), scale * (a.x * hv.y + a.y * hv.x));
   }
   fft_stages<N, E, 1, +1>(v, t, L, tw, typename Plan<N, E>::Rad{});
+  {
+    float2 *p = s0;
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int z = t + T * e;
-    if (z < g.iz) S[z * g.sz + col] = v[e];
+    for (int e = 0; e < E; e++, p += step)
+      if (t + T * e < g.iz) *p = v[e];
   }
 }
 
 // ------------------------------------------------------------ dispatch
 
-enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KNUM };
-const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod", "y_inv",
-                                  "x_mid", "x_last", "finish"};
+enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KX_LAST_FWD, KNUM };
+const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod",  "y_inv",
+                                  "x_mid", "x_last", "finish", "x_last_fwd"};
 
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
 // Launch shape of the fused kernels; defaults per size, overridable with
-// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zp=0"
+// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zt=16,fx=1"
 // (elements per thread and tile rows of x_fwd, x_mid, x_last; y pencils per
-// workgroup; z elements per thread; z OTF prefetch).
+// workgroup; z elements per thread and tile columns; x_last_fwd chaining).
 struct Tune {
   int xe[3] = {0, 0, 0};     // x_fwd, x_mid, x_last: elements per thread
   int xt[3] = {16, 16, 16};  // rows per tile
-  int yp = 0, ze = 0, zp = -1;
+  int yp = 0, ze = 0, zt = 16;
+  int fx = 1;                // chain x_last into the next x_fwd
 };
 
 bool x_supported(int M) {
@@ -674,6 +775,9 @@ void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   else if (kind == KX_MID)
     hipLaunchKernelGGL((k_x_mid<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
+  else if (kind == KX_LAST_FWD)
+    hipLaunchKernelGGL((k_x_last_fwd<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+                       b.scal + 2, b.parts, b.twM, b.twN, g);
   else
     hipLaunchKernelGGL((k_x_last<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
@@ -753,40 +857,50 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
   }
 }
 
-template <int N, int E, bool PRE>
-void launch_z_e(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
-  const unsigned grid = (unsigned)(g.hx * (g.ny / kTPz));
-  hipLaunchKernelGGL((k_z<N, E, kTPz, PRE>), dim3(grid), dim3(kTPz * (N / E)), 0, s,
-                     b.S, b.otf, b.twZ, g, scale);
+template <int N, int E, int TP>
+void launch_z_t(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  const unsigned grid = (unsigned)(g.hx * (g.ny / TP));
+  hipLaunchKernelGGL((k_z<N, E, TP>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.otf,
+                     b.twZ, g, scale);
+}
+
+// Tile widths 32 and 64 where the workgroup stays within 1024 threads and
+// its tile within 128 KB of LDS, and ny is a multiple of the width.
+template <int N, int E>
+void launch_z_e(int tp, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+  constexpr bool ok32 = 32 * (N / E) <= 1024 && 32 * N * 8 <= 131072;
+  constexpr bool ok64 = 64 * (N / E) <= 1024 && 64 * N * 8 <= 131072;
+  if constexpr (ok64) {
+    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(g, b, scale, s);
+  }
+  if constexpr (ok32) {
+    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(g, b, scale, s);
+  }
+  launch_z_t<N, E, kTPz>(g, b, scale, s);
 }
 
 template <int N>
-void launch_z(int e, int pre, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+void launch_z(int e, int tp, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   if constexpr (N % 3 == 0) {
-    if (pre) launch_z_e<N, 12, true>(g, b, scale, s);
-    else launch_z_e<N, 12, false>(g, b, scale, s);
+    launch_z_e<N, 12>(tp, g, b, scale, s);
   } else if constexpr (N == 64) {
-    if (pre) launch_z_e<N, 8, true>(g, b, scale, s);
-    else launch_z_e<N, 8, false>(g, b, scale, s);
+    launch_z_e<N, 8>(tp, g, b, scale, s);
   } else {
-    if (e == 8) {
-      if (pre) launch_z_e<N, 8, true>(g, b, scale, s);
-      else launch_z_e<N, 8, false>(g, b, scale, s);
-    } else {
-      if (pre) launch_z_e<N, 16, true>(g, b, scale, s);
-      else launch_z_e<N, 16, false>(g, b, scale, s);
-    }
+    if (e == 8)
+      launch_z_e<N, 8>(tp, g, b, scale, s);
+    else
+      launch_z_e<N, 16>(tp, g, b, scale, s);
   }
 }
 
 void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   switch (g.nz) {
-    case 64: launch_z<64>(tu.ze, tu.zp, g, b, scale, s); break;
-    case 96: launch_z<96>(tu.ze, tu.zp, g, b, scale, s); break;
-    case 128: launch_z<128>(tu.ze, tu.zp, g, b, scale, s); break;
-    case 192: launch_z<192>(tu.ze, tu.zp, g, b, scale, s); break;
-    case 256: launch_z<256>(tu.ze, tu.zp, g, b, scale, s); break;
-    case 512: launch_z<512>(tu.ze, tu.zp, g, b, scale, s); break;
+    case 64: launch_z<64>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 96: launch_z<96>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 128: launch_z<128>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 192: launch_z<192>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 256: launch_z<256>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 512: launch_z<512>(tu.ze, tu.zt, g, b, scale, s); break;
   }
 }
 
@@ -801,7 +915,7 @@ Tune pick_tune(const Geom &g) {
   if (g.nx / 2 == 512) t.xt[1] = t.xt[2] = 8;  // measured: x_mid, x_last
   t.yp = 4;
   t.ze = (g.nz == 128) ? 8 : 16;
-  t.zp = 0;
+  t.zt = (g.nz >= 256) ? 32 : 16;  // measured: 256-byte plane segments at nz 256
   if (const char *s = std::getenv("CLARITY_FAST_TUNE")) {
     int v;
     const char *p = s;
@@ -814,7 +928,8 @@ Tune pick_tune(const Geom &g) {
       else if (std::sscanf(p, "lt=%d", &v) == 1) t.xt[2] = v;
       else if (std::sscanf(p, "yp=%d", &v) == 1) t.yp = v;
       else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
-      else if (std::sscanf(p, "zp=%d", &v) == 1) t.zp = v;
+      else if (std::sscanf(p, "zt=%d", &v) == 1) t.zt = v;
+      else if (std::sscanf(p, "fx=%d", &v) == 1) t.fx = v;
       while (*p && *p != ',') p++;
       if (*p == ',') p++;
     }
@@ -836,7 +951,8 @@ class FastRL : public RLEngine {
     nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
   }
   ~FastRL() override {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    for (auto &ge : graph_exec_)
+      if (ge) (void)hipGraphExecDestroy(ge);
     for (auto &ev : ev_pool_) {
       (void)hipEventDestroy(ev.first);
       (void)hipEventDestroy(ev.second);
@@ -911,44 +1027,65 @@ class FastRL : public RLEngine {
     return CLARITY_SUCCESS;
   }
 
-  // One RL iteration = ten launches.  Without per-kernel timing the
-  // sequence is captured once into a HIP graph and replayed, which removes
-  // the host launch cost (it matters for small volumes).  The graph holds
-  // kernel nodes only; every buffer it touches lives as long as the engine.
+  // n RL iterations.  Chained (the default): x_fwd, then n bodies; every
+  // body but the last ends in x_last_fwd, which also makes the next
+  // iteration's x pass, the last in x_last.  Unchained (tune fx=0): n times
+  // x_fwd + body.  Without per-kernel timing the bodies are captured once
+  // into HIP graphs and replayed, which removes the host launch cost (it
+  // matters for small volumes).  The graphs hold kernel nodes only; every
+  // buffer they touch lives as long as the engine.
   ClarityResult_t iterate(int n, hipStream_t s) override {
     if (n <= 0) return CLARITY_SUCCESS;
+    const bool chain = tu_.fx != 0;
     if (timing_ || !use_graph_) {
-      for (int it = 0; it < n; it++) CL_TRY(enqueue_iteration(s));
+      for (int it = 0; it < n; it++) {
+        if (it == 0 || !chain) CL_TRY(run(KX_FWD, s));
+        CL_TRY(enqueue_body(chain && it + 1 < n, s));
+      }
       return CLARITY_SUCCESS;
     }
-    if (!graph_exec_) {
-      hipGraph_t graph = nullptr;
-      CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      ClarityResult_t r = enqueue_iteration(s);
-      hipError_t e = hipStreamEndCapture(s, &graph);
-      if (r != CLARITY_SUCCESS) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return r;
-      }
-      CL_HIP(e);
-      e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) {
-        graph_exec_ = nullptr;
-        return from_hip(e);
-      }
+    if (!chain) {
+      if (!graph_exec_[2]) CL_TRY(capture(2, &graph_exec_[2], s));
+      for (int it = 0; it < n; it++) CL_HIP(hipGraphLaunch(graph_exec_[2], s));
+      return CLARITY_SUCCESS;
     }
-    for (int it = 0; it < n; it++) CL_HIP(hipGraphLaunch(graph_exec_, s));
+    CL_TRY(run(KX_FWD, s));
+    for (int k = 0; k < 2; k++)
+      if (!graph_exec_[k]) CL_TRY(capture(k, &graph_exec_[k], s));
+    for (int it = 0; it + 1 < n; it++) CL_HIP(hipGraphLaunch(graph_exec_[0], s));
+    CL_HIP(hipGraphLaunch(graph_exec_[1], s));
     return CLARITY_SUCCESS;
   }
 
-  ClarityResult_t enqueue_iteration(hipStream_t s) {
-    CL_TRY(run(KX_FWD, s));
+  // kind 0: chained body, 1: last body, 2: x_fwd + unchained body
+  ClarityResult_t capture(int kind, hipGraphExec_t *out, hipStream_t s) {
+    hipGraph_t graph = nullptr;
+    CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    ClarityResult_t r = (kind == 2) ? run(KX_FWD, s) : CLARITY_SUCCESS;
+    if (r == CLARITY_SUCCESS) r = enqueue_body(kind == 0, s);
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (r != CLARITY_SUCCESS) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    CL_HIP(e);
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      *out = nullptr;
+      return from_hip(e);
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  // One iteration after its x_fwd: nine launches (the last x pass fused
+  // with the next iteration's x_fwd when `chained`).
+  ClarityResult_t enqueue_body(bool chained, hipStream_t s) {
     for (int half = 0; half < 2; half++) {
       CL_TRY(run(KY_FWD, s));
       CL_TRY(run(KZ_MOD, s));
       CL_TRY(run(KY_INV, s));
-      CL_TRY(run(half == 0 ? KX_MID : KX_LAST, s));
+      CL_TRY(run(half == 0 ? KX_MID : chained ? KX_LAST_FWD : KX_LAST, s));
     }
     return run(KFINISH, s);
   }
@@ -975,8 +1112,11 @@ class FastRL : public RLEngine {
         c * Sp + f * IV + c * Sp,       // x_mid
         c * Sp + f * IV + f * IV,       // x_last
         8.0 * nparts_ + 12.0,           // finish
+        2 * c * Sp + 2 * f * IV,        // x_last_fwd
     };
-    const int per_it[KNUM] = {1, 2, 2, 2, 1, 1, 1};
+    // launches per iteration of a long run: x_fwd and x_last run once per
+    // iterate call, x_last_fwd joins the iterations in between
+    const int per_it[KNUM] = {0, 2, 2, 2, 1, 0, 1, 1};
     int n = 0;
     for (int k = 0; k < KNUM && n < cap; k++, n++) {
       names[n] = kNames[k];
@@ -1035,7 +1175,8 @@ class FastRL : public RLEngine {
     switch (kind) {
       case KX_FWD:
       case KX_MID:
-      case KX_LAST: dispatch_x(tu_, kind, g_, b_, s); break;
+      case KX_LAST:
+      case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
       case KZ_MOD: dispatch_z(tu_, g_, b_, scale, s); break;
@@ -1055,7 +1196,7 @@ class FastRL : public RLEngine {
   Geom g_{};
   Tune tu_{};
   bool use_graph_ = std::getenv("CLARITY_NO_GRAPH") == nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};  // see capture()
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, otf_, parts_, scal_;

@@ -95,7 +95,8 @@ C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanDestroy(ClarityEx_RLPlan plan);
 
 /* Describes the plan's kernels: writes up to `cap` entries, returns count.
  * names[i] is a static string, bytes[i] the algorithmic HBM bytes one launch
- * of that kernel moves, launches[i] its launches per RL iteration. */
+ * of that kernel moves, launches[i] its launches per RL iteration of a long
+ * run (kernels that run once per ClarityEx_RLPlanIterate call report 0). */
 C_FUNC_DEF int ClarityEx_RLPlanDescribe(ClarityEx_RLPlan plan,
                                         const char **names, double *bytes,
                                         int *launches, int cap);

@@ -142,14 +142,35 @@ def test_fast_timing_api(clarity_gpu):
     plan.set_inputs(case.obs, case.psf)
     kinds = plan.describe()
     assert [k[0] for k in kinds] == ["x_fwd", "y_fwd", "z_mod", "y_inv", "x_mid",
-                                     "x_last", "finish"]
+                                     "x_last", "finish", "x_last_fwd"]
+    assert [k[2] for k in kinds] == [0, 2, 2, 2, 1, 0, 1, 1]
     plan.set_timing(True)
     plan.timings(reset=True)
     plan.iterate(3)
+    plan.iterate(1)
     t = plan.timings()
-    assert [c for c, _ in t] == [3 * k[2] for k in kinds]
+    # x_fwd and x_last once per call; x_last_fwd joins iterations of a call
+    assert [c for c, _ in t] == [2, 8, 8, 8, 4, 2, 4, 2]
     assert all(ms > 0 for _, ms in t)
     plan.close()
+
+
+def test_fused_x_boundary_matches_split(clarity_gpu):
+    """Iterations chained through x_last_fwd equal the same iterations run
+    one call at a time (x_last, then x_fwd), bit for bit."""
+    api = clarity_gpu
+    case = synth.make_config(1)
+    plan = api.RLPlan(case.obs.shape, case.psf.shape)
+    plan.set_inputs(case.obs, case.psf)
+    plan.reset()
+    plan.iterate(5)
+    chained = plan.estimate()
+    plan.reset()
+    for _ in range(5):
+        plan.iterate(1)
+    split = plan.estimate()
+    plan.close()
+    assert np.array_equal(chained, split)
 
 
 def test_batch_uneven_volumes(clarity_gpu):
