@@ -307,10 +307,35 @@ struct Geom {
 };
 
 // Copies a twiddle table into LDS (cooperatively; caller synchronises).
-template <int NT>
-__device__ __forceinline__ void stage_table(float2 *dst, const float2 *__restrict__ src,
-                                            int n) {
-  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+// A table of N entries moved from global memory to LDS by NT threads in two
+// phases, load() into registers and store() to LDS, so that a kernel can
+// issue all its loads before its first LDS store: their latencies then
+// overlap instead of adding up (the vector-memory counter is in order).
+template <int NT, int N>
+struct TableRegs {
+  static constexpr int IT = (N + NT - 1) / NT;
+  float2 t[IT];
+  // entry i = threadIdx.x + it * NT exists for every it but possibly the last
+  __device__ __forceinline__ static bool has(int it) {
+    return (it + 1) * NT <= N || (int)threadIdx.x + it * NT < N;
+  }
+  __device__ __forceinline__ void load(const float2 *__restrict__ src) {
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+      if (has(it)) t[it] = src[threadIdx.x + it * NT];
+  }
+  __device__ __forceinline__ void store(float2 *dst) const {
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+      if (has(it)) dst[threadIdx.x + it * NT] = t[it];
+  }
+};
+
+template <int NT, int N>
+__device__ __forceinline__ void stage_table(float2 *dst, const float2 *__restrict__ src) {
+  TableRegs<NT, N> r;
+  r.load(src);
+  r.store(dst);
 }
 
 // ------------------------------------------------------------- x passes
@@ -346,24 +371,47 @@ __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restri
 // Loads X[0..M] of TP rows from S into LDS (row layout); the real parts of
 // X[0] and X[M] share slot 0 (their imaginary parts are ignored, as a
 // multi-dimensional C2R ignores them).
-template <int M, int TP, int NT, int PS>
-__device__ __forceinline__ void load_spectrum(float2 *lds, const float2 *__restrict__ S,
-                                              const Geom &g, int z, int y0) {
-  constexpr int MP = kRowStride<M, PS>;
-  for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
-    const int qq = idx % TP;
-    const int k = idx / TP;
-    float2 X = make_float2(0.f, 0.f);
-    if (y0 + qq < g.iy) X = S[s_index(g, z, k, y0 + qq)];
-    float2 *row = lds + qq * MP;
-    if (k == 0)
-      row[0].x = X.x;
-    else if (k == M)
-      row[0].y = X.x;
-    else
-      row[padded<PS>(k)] = X;
+// Thread (qq, k0) = (threadIdx % TP, threadIdx / TP) handles row qq at
+// k = k0, k0 + NT/TP, ...; load() issues all its loads, store() writes them
+// to LDS (the two phases as in TableRegs).
+template <int M, int TP, int NT>
+struct SpectrumRegs {
+  static_assert(NT % TP == 0, "a thread keeps one row");
+  static constexpr int KS = NT / TP;
+  static constexpr int IT = (M + 1 + KS - 1) / KS;
+  float2 X[IT];
+  __device__ __forceinline__ void load(const float2 *__restrict__ S, const Geom &g, int z,
+                                       int y0) {
+    const int qq = threadIdx.x % TP;
+    const int k0 = threadIdx.x / TP;
+    // unconditional loads (row and k clamped): no branches between them, so
+    // all are in flight together.  No value needs masking: entries k > M are
+    // never stored, and a row past the image (a partial tile) is transformed
+    // on its own and never written back.
+    const float2 *src = S + s_index(g, z, 0, min(y0 + qq, g.iy - 1));
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int k = k0 + it * KS;  // <= M except possibly in the last step
+      X[it] = src[(long long)(it + 1 < IT ? k : min(k, M)) * g.ny];
+    }
   }
-}
+  template <int PS>
+  __device__ __forceinline__ void store(float2 *lds) const {
+    const int qq = threadIdx.x % TP;
+    const int k0 = threadIdx.x / TP;
+    float2 *row = lds + qq * kRowStride<M, PS>;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int k = k0 + it * KS;
+      if (k == 0)
+        row[0].x = X[it].x;
+      else if (k == M)
+        row[0].y = X[it].x;
+      else if (k < M)
+        row[padded<PS>(k)] = X[it];
+    }
+  }
+};
 
 // Forms Zc[k] = (X[k] + conj X[M-k]) + i w^-k (X[k] - conj X[M-k]) for the
 // thread's k = t + T*e; its inverse FFT is x[2m] + i x[2m+1].
@@ -392,16 +440,50 @@ __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const L_t &L, int t
   }
 }
 
-// Row loads of a real image row pair (x[2m], x[2m+1]) for m = t + T*e.
+// Row loads of a real image row pair (x[2m], x[2m+1]) for m = t + T*e.  With
+// ix even every row starts 8-byte aligned and a pair is one 8-byte load.
+// The loads are unconditional (addresses clamped into the row, rows of
+// invalid tiles clamped by the caller) and the values outside the image
+// zeroed by a 0/1 factor (the clamped loads read finite image values), not
+// by a select, which the compiler would turn back into a branch around the
+// load: without branches between them every load is in flight together.
 template <int E, int T>
 __device__ __forceinline__ void load_row_pairs(float2 (&v)[E], const float *row, bool valid,
                                                int t, int ix) {
+  if ((ix & 1) == 0) {
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int n0 = 2 * (t + T * e);
-    const float a = (valid && n0 < ix) ? row[n0] : 0.f;
-    const float b = (valid && n0 + 1 < ix) ? row[n0 + 1] : 0.f;
-    v[e] = make_float2(a, b);
+    for (int e = 0; e < E; e++) {
+      const int n0 = 2 * (t + T * e);
+      const float2 p = *reinterpret_cast<const float2 *>(row + min(n0, ix - 2));
+      const float m = (valid && n0 < ix) ? 1.f : 0.f;
+      v[e] = make_float2(p.x * m, p.y * m);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int n0 = 2 * (t + T * e);
+      const float a = row[min(n0, ix - 1)];
+      const float b = row[min(n0 + 1, ix - 1)];
+      v[e] = make_float2(a * ((valid && n0 < ix) ? 1.f : 0.f),
+                         b * ((valid && n0 + 1 < ix) ? 1.f : 0.f));
+    }
+  }
+}
+
+// Image row y of plane z for loads; rows past the image (partial tiles) are
+// clamped onto the last one, whose values the loads then discard.
+__device__ __forceinline__ const float *load_row(const float *base, const Geom &g, int z,
+                                                 int y) {
+  return base + ((long long)z * g.iy + min(y, g.iy - 1)) * g.ix;
+}
+
+// Stores (a, b) to row[n0], row[n0 + 1] (inside the row's ix entries).
+__device__ __forceinline__ void store_row_pair(float *row, int n0, int ix, float a, float b) {
+  if ((ix & 1) == 0) {
+    *reinterpret_cast<float2 *>(row + n0) = make_float2(a, b);
+  } else {
+    row[n0] = a;
+    if (n0 + 1 < ix) row[n0 + 1] = b;
   }
 }
 
@@ -447,10 +529,14 @@ __global__ void __launch_bounds__(TP *(M / E))
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
   const int y = y0 + q;
+  TableRegs<NT, M> tm;
+  TableRegs<NT, M + 1> tn;
+  tm.load(gtwM);
+  tn.load(gtwN);
   float2 v[E];
-  load_row_pairs<E, T>(v, u + ((long long)z * g.iy + y) * g.ix, y < g.iy, t, g.ix);
-  stage_table<NT>(sm.twM, gtwM, M);
-  stage_table<NT>(sm.twN, gtwN, M + 1);
+  load_row_pairs<E, T>(v, load_row(u, g, z, y), y < g.iy, t, g.ix);
+  tm.store(sm.twM);
+  tn.store(sm.twN);
   __syncthreads();
   const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
@@ -477,13 +563,20 @@ __global__ void __launch_bounds__(TP *(M / E))
   const int y0 = (tile % ntile) * TP;
   const int y = y0 + q;
   const bool valid = y < g.iy;
-  // the observation row is needed only after the inverse transform; its
-  // loads are issued first so they overlap the spectrum load and the FFT
+  // every load of the workgroup is in flight before the first LDS store;
+  // the observation row is needed only after the inverse transform, so its
+  // loads go last and the stores wait for the spectrum and tables only
+  SpectrumRegs<M, TP, NT> sp;
+  sp.load(S, g, z, y0);
+  TableRegs<NT, M> tm;
+  TableRegs<NT, M + 1> tn;
+  tm.load(gtwM);
+  tn.load(gtwN);
   float2 o[E];
-  load_row_pairs<E, T>(o, obs + ((long long)z * g.iy + y) * g.ix, valid, t, g.ix);
-  stage_table<NT>(sm.twM, gtwM, M);
-  stage_table<NT>(sm.twN, gtwN, M + 1);
-  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
+  load_row_pairs<E, T>(o, load_row(obs, g, z, y), valid, t, g.ix);
+  sp.template store<PS>(sm.tile);
+  tm.store(sm.twM);
+  tn.store(sm.twN);
   __syncthreads();
   const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   float2 v[E];
@@ -534,11 +627,18 @@ __global__ void __launch_bounds__(TP *(M / E))
   const int y = y0 + q;
   const bool valid = y < g.iy;
   float *urow = u + ((long long)z * g.iy + y) * g.ix;
+  // loads in the order they are consumed (see k_x_mid)
+  SpectrumRegs<M, TP, NT> sp;
+  sp.load(S, g, z, y0);
+  TableRegs<NT, M> tm;
+  TableRegs<NT, M + 1> tn;
+  tm.load(gtwM);
+  tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T>(uo, urow, valid, t, g.ix);
-  stage_table<NT>(sm.twM, gtwM, M);
-  stage_table<NT>(sm.twN, gtwN, M + 1);
-  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
+  load_row_pairs<E, T>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  sp.template store<PS>(sm.tile);
+  tm.store(sm.twM);
+  tn.store(sm.twN);
   __syncthreads();
   const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   float2 v[E];
@@ -553,14 +653,12 @@ __global__ void __launch_bounds__(TP *(M / E))
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     if (valid && n0 < g.ix) {
-      const float un = (c * uo[e].x) * v[e].x;
-      urow[n0] = un;
-      acc += (double)un;
-    }
-    if (valid && n0 + 1 < g.ix) {
-      const float un = (c * uo[e].y) * v[e].y;
-      urow[n0 + 1] = un;
-      acc += (double)un;
+      const bool second = n0 + 1 < g.ix;
+      const float u0 = (c * uo[e].x) * v[e].x;
+      const float u1 = second ? (c * uo[e].y) * v[e].y : 0.f;
+      store_row_pair(urow, n0, g.ix, u0, u1);
+      acc += (double)u0;
+      if (second) acc += (double)u1;
     }
   }
   double *sh = reinterpret_cast<double *>(sm.tile);
@@ -597,11 +695,18 @@ __global__ void __launch_bounds__(TP *(M / E))
   const int y = y0 + q;
   const bool valid = y < g.iy;
   float *urow = u + ((long long)z * g.iy + y) * g.ix;
+  // loads in the order they are consumed (see k_x_mid)
+  SpectrumRegs<M, TP, NT> sp;
+  sp.load(S, g, z, y0);
+  TableRegs<NT, M> tm;
+  TableRegs<NT, M + 1> tn;
+  tm.load(gtwM);
+  tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T>(uo, urow, valid, t, g.ix);
-  stage_table<NT>(sm.twM, gtwM, M);
-  stage_table<NT>(sm.twN, gtwN, M + 1);
-  load_spectrum<M, TP, NT, PS>(sm.tile, S, g, z, y0);
+  load_row_pairs<E, T>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  sp.template store<PS>(sm.tile);
+  tm.store(sm.twM);
+  tn.store(sm.twN);
   __syncthreads();
   const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
   float2 v[E];
@@ -616,14 +721,12 @@ __global__ void __launch_bounds__(TP *(M / E))
     const int n0 = 2 * (t + T * e);
     float u0 = 0.f, u1 = 0.f;
     if (valid && n0 < g.ix) {
+      const bool second = n0 + 1 < g.ix;
       u0 = (c * uo[e].x) * v[e].x;
-      urow[n0] = u0;
+      if (second) u1 = (c * uo[e].y) * v[e].y;
+      store_row_pair(urow, n0, g.ix, u0, u1);
       acc += (double)u0;
-    }
-    if (valid && n0 + 1 < g.ix) {
-      u1 = (c * uo[e].y) * v[e].y;
-      urow[n0 + 1] = u1;
-      acc += (double)u1;
+      if (second) acc += (double)u1;
     }
     v[e] = make_float2(u0, u1);
   }
@@ -645,7 +748,10 @@ __global__ void __launch_bounds__(TP *(M / E))
 
 // ------------------------------------------------------------- y and z
 
-template <int N, int E, int PPW, int SIGN>
+// H: the image is half the work domain along y (iy = N/2, a same-size
+// kernel), so the forward transform's nonzero inputs and the inverse's kept
+// outputs are exactly elements e < E/2 of every thread: no bounds tests.
+template <int N, int E, int PPW, int SIGN, bool H>
 __global__ void __launch_bounds__(PPW *(N / E))
     k_y(float2 *__restrict__ S, const float2 *__restrict__ gtw, long long npencils,
         int Lin, int Lout) {
@@ -661,26 +767,41 @@ __global__ void __launch_bounds__(PPW *(N / E))
   const long long p = (long long)blockIdx.x * PPW + pl;
   const bool valid = p < npencils;
   float2 *base = S + p * N;
+  // loads without branches (pencil and index clamped, value selected), so
+  // that all of them are in flight together; a pencil past the end
+  // transforms a copy of the last one and stores nothing
+  const float2 *lbase = S + (valid ? p : npencils - 1) * N;
   float2 v[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int j = t + T * e;
-    v[e] = (valid && j < Lin) ? base[j] : make_float2(0.f, 0.f);
+    if constexpr (H && SIGN < 0) {
+      v[e] = (e < E / 2) ? lbase[j] : make_float2(0.f, 0.f);
+    } else {
+      const float2 x = lbase[min(j, Lin - 1)];
+      v[e] = (j < Lin) ? x : make_float2(0.f, 0.f);
+    }
   }
-  stage_table<NT>(tw, gtw, N);
+  stage_table<NT, N>(tw, gtw);
   __syncthreads();
   const PencilLds<(T <= 64), PS> L{lds + pl * NP};
   fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename Plan<N, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int j = t + T * e;
-    if (valid && j < Lout) base[j] = v[e];
+    if constexpr (H && SIGN > 0) {
+      if (e < E / 2 && valid) base[j] = v[e];
+    } else {
+      if (valid && j < Lout) base[j] = v[e];
+    }
   }
 }
 
 // A workgroup transforms TP adjacent ky columns of one kx (TP * 8 bytes
-// contiguous per z plane); TP lanes of a wavefront read one plane.
-template <int N, int E, int TP>
+// contiguous per z plane); TP lanes of a wavefront read one plane.  H: the
+// image is half the work domain along z (iz = N/2), so the planes read and
+// written are exactly elements e < E/2: no bounds tests.
+template <int N, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
         const float2 *__restrict__ gtw, Geom g, float scale) {
@@ -701,10 +822,14 @@ __global__ void __launch_bounds__(TP *(N / E))
   {
     const float2 *p = s0;
 #pragma unroll
-    for (int e = 0; e < E; e++, p += step)
-      v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
+    for (int e = 0; e < E; e++, p += step) {
+      if constexpr (H)
+        v[e] = (e < E / 2) ? *p : make_float2(0.f, 0.f);
+      else
+        v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
+    }
   }
-  stage_table<NT>(tw, gtw, N);
+  stage_table<NT, N>(tw, gtw);
   __syncthreads();
   const InterLds<TP> L{lds + q};
   fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
@@ -719,8 +844,13 @@ __global__ void __launch_bounds__(TP *(N / E))
   {
     float2 *p = s0;
 #pragma unroll
-    for (int e = 0; e < E; e++, p += step)
-      if (t + T * e < g.iz) *p = v[e];
+    for (int e = 0; e < E; e++, p += step) {
+      if constexpr (H) {
+        if (e < E / 2) *p = v[e];
+      } else {
+        if (t + T * e < g.iz) *p = v[e];
+      }
+    }
   }
 }
 
@@ -823,12 +953,22 @@ template <int N, int E, int PPW>
 void launch_y_p(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   const long long np = (long long)g.iz * g.hx;
   const unsigned grid = (unsigned)((np + PPW - 1) / PPW);
-  if (fwd)
-    hipLaunchKernelGGL((k_y<N, E, PPW, -1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
-                       b.twY, np, g.iy, N);
-  else
-    hipLaunchKernelGGL((k_y<N, E, PPW, +1>), dim3(grid), dim3(PPW * (N / E)), 0, s, b.S,
-                       b.twY, np, N, g.iy);
+  const dim3 blk(PPW * (N / E));
+  if (2 * g.iy == N) {
+    if (fwd)
+      hipLaunchKernelGGL((k_y<N, E, PPW, -1, true>), dim3(grid), blk, 0, s, b.S, b.twY, np,
+                         g.iy, N);
+    else
+      hipLaunchKernelGGL((k_y<N, E, PPW, +1, true>), dim3(grid), blk, 0, s, b.S, b.twY, np,
+                         N, g.iy);
+  } else {
+    if (fwd)
+      hipLaunchKernelGGL((k_y<N, E, PPW, -1, false>), dim3(grid), blk, 0, s, b.S, b.twY, np,
+                         g.iy, N);
+    else
+      hipLaunchKernelGGL((k_y<N, E, PPW, +1, false>), dim3(grid), blk, 0, s, b.S, b.twY, np,
+                         N, g.iy);
+  }
 }
 
 template <int N>
@@ -860,8 +1000,12 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
 template <int N, int E, int TP>
 void launch_z_t(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   const unsigned grid = (unsigned)(g.hx * (g.ny / TP));
-  hipLaunchKernelGGL((k_z<N, E, TP>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.otf,
-                     b.twZ, g, scale);
+  if (2 * g.iz == N)
+    hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
+                       b.otf, b.twZ, g, scale);
+  else
+    hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
+                       b.otf, b.twZ, g, scale);
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
@@ -912,7 +1056,7 @@ Tune pick_tune(const Geom &g) {
     t.xe[j] = xe;
     t.xt[j] = 16;
   }
-  if (g.nx / 2 == 512) t.xt[1] = t.xt[2] = 8;  // measured: x_mid, x_last
+  if (g.nx / 2 == 512) t.xt[0] = t.xt[1] = t.xt[2] = 8;  // measured at 512x512x128
   t.yp = 4;
   t.ze = (g.nz == 128) ? 8 : 16;
   t.zt = (g.nz >= 256) ? 32 : 16;  // measured: 256-byte plane segments at nz 256
