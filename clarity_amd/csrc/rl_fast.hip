@@ -818,6 +818,13 @@ __global__ void __launch_bounds__(TP *(N / E))
   const long long step = (long long)T * g.sz;
   float2 *const s0 = S + (long long)t * g.sz + col;
   const float2 *const h0 = otf + (long long)t * g.sz + col;
+  // With 32 or more columns the tile's LDS already limits a CU to 16
+  // wavefronts, so registers are free up to 128: the OTF column is then
+  // loaded up front, behind the twiddles and the spectrum (the counter of
+  // outstanding loads is in order), and arrives during the forward FFT.
+  constexpr bool PRE = TP >= 32;
+  TableRegs<NT, N> twr;
+  twr.load(gtw);
   float2 v[E];
   {
     const float2 *p = s0;
@@ -829,14 +836,23 @@ __global__ void __launch_bounds__(TP *(N / E))
         v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
     }
   }
-  stage_table<NT, N>(tw, gtw);
+  float2 h[PRE ? E : 1];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int e = 0; e < E; e++) h[e] = h0[e * step];
+  }
+  twr.store(tw);
   __syncthreads();
   const InterLds<TP> L{lds + q};
   fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
   // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    const float2 hv = h0[e * step];
+    float2 hv;
+    if constexpr (PRE)
+      hv = h[e];
+    else
+      hv = h0[e * step];
     const float2 a = v[e];
     v[e] = make_float2(scale * (a.x * hv.x - a.y * hv.y), scale * (a.x * hv.y + a.y * hv.x));
   }
