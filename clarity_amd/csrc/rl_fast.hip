@@ -447,10 +447,37 @@ __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const L_t &L, int t
 // zeroed by a 0/1 factor (the clamped loads read finite image values), not
 // by a select, which the compiler would turn back into a branch around the
 // load: without branches between them every load is in flight together.
-template <int E, int T>
+// Element e of a thread holds the real positions n0 = 2 (t + T e), n0 + 1.
+// With H (ix == M, a same-size kernel along x) exactly the elements e < E/2
+// lie in the image, known at compile time.
+template <bool H, int E>
+__device__ __forceinline__ bool in_image(int e, int n0, int ix) {
+  if constexpr (H)
+    return e < E / 2;
+  else
+    return n0 < ix;
+}
+template <bool H, int E>
+__device__ __forceinline__ bool second_in_image(int e, int n0, int ix) {
+  if constexpr (H)
+    return e < E / 2;
+  else
+    return n0 + 1 < ix;
+}
+
+template <int E, int T, bool H = false>
 __device__ __forceinline__ void load_row_pairs(float2 (&v)[E], const float *row, bool valid,
                                                int t, int ix) {
-  if ((ix & 1) == 0) {
+  if constexpr (H) {
+    // rows of a partial tile are transformed alone and never stored, so
+    // they need no masking; the elements past the image are zero
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      v[e] = (e < E / 2) ? *reinterpret_cast<const float2 *>(row + 2 * (t + T * e))
+                         : make_float2(0.f, 0.f);
+    (void)valid;
+    (void)ix;
+  } else if ((ix & 1) == 0) {
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int n0 = 2 * (t + T * e);
@@ -514,7 +541,7 @@ struct XSmem {
   float2 twN[M + 1];
 };
 
-template <int M, int E, int TP>
+template <int M, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
@@ -534,7 +561,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 v[E];
-  load_row_pairs<E, T>(v, load_row(u, g, z, y), y < g.iy, t, g.ix);
+  load_row_pairs<E, T, H>(v, load_row(u, g, z, y), y < g.iy, t, g.ix);
   tm.store(sm.twM);
   tn.store(sm.twN);
   __syncthreads();
@@ -546,7 +573,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-template <int M, int E, int TP>
+template <int M, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
             const float *__restrict__ cptr, const float2 *__restrict__ gtwM,
@@ -573,7 +600,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 o[E];
-  load_row_pairs<E, T>(o, load_row(obs, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, H>(o, load_row(obs, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -592,11 +619,11 @@ __global__ void __launch_bounds__(TP *(M / E))
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     float r0 = 0.f, r1 = 0.f;
-    if (valid && n0 < g.ix) {
+    if (valid && in_image<H, E>(e, n0, g.ix)) {
       const float s = c * v[e].x;
       r0 = (fabsf(s) < kDivGuard) ? 0.f : o[e].x / s;
     }
-    if (valid && n0 + 1 < g.ix) {
+    if (valid && second_in_image<H, E>(e, n0, g.ix)) {
       const float s = c * v[e].y;
       r1 = (fabsf(s) < kDivGuard) ? 0.f : o[e].y / s;
     }
@@ -609,7 +636,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-template <int M, int E, int TP>
+template <int M, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
@@ -635,7 +662,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, H>(uo, load_row(u, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -652,8 +679,8 @@ __global__ void __launch_bounds__(TP *(M / E))
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
-    if (valid && n0 < g.ix) {
-      const bool second = n0 + 1 < g.ix;
+    if (valid && in_image<H, E>(e, n0, g.ix)) {
+      const bool second = second_in_image<H, E>(e, n0, g.ix);
       const float u0 = (c * uo[e].x) * v[e].x;
       const float u1 = second ? (c * uo[e].y) * v[e].y : 0.f;
       store_row_pair(urow, n0, g.ix, u0, u1);
@@ -677,7 +704,7 @@ __global__ void __launch_bounds__(TP *(M / E))
 // along x (the next iteration's first pass, which needs no scale: c enters
 // at x_mid) goes to S.  The tile's spectrum rows are read completely before
 // any of them is rewritten, and no other workgroup touches them.
-template <int M, int E, int TP>
+template <int M, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
                  const float *__restrict__ cptr, double *__restrict__ partials,
@@ -703,7 +730,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, H>(uo, load_row(u, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -720,8 +747,8 @@ __global__ void __launch_bounds__(TP *(M / E))
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     float u0 = 0.f, u1 = 0.f;
-    if (valid && n0 < g.ix) {
-      const bool second = n0 + 1 < g.ix;
+    if (valid && in_image<H, E>(e, n0, g.ix)) {
+      const bool second = second_in_image<H, E>(e, n0, g.ix);
       u0 = (c * uo[e].x) * v[e].x;
       if (second) u1 = (c * uo[e].y) * v[e].y;
       store_row_pair(urow, n0, g.ix, u0, u1);
@@ -887,6 +914,9 @@ struct Tune {
   int xt[3] = {16, 16, 16};  // rows per tile
   int yp = 0, ze = 0, zt = 16;
   int fx = 1;                // chain x_last into the next x_fwd
+  int xh = 7;                // bit j: same-size specialisation of x kernel j
+                             // (x_fwd, x_mid, x_last, x_last_fwd); measured
+                             // slower for x_last_fwd at 512x512x128
 };
 
 bool x_supported(int M) {
@@ -910,58 +940,66 @@ struct Bufs {
 // Number of x tiles (= x_last partial sums) for a tile height.
 int x_tiles(const Geom &g, int tp) { return g.iz * ((g.iy + tp - 1) / tp); }
 
-template <int M, int E, int TP>
+template <int M, int E, int TP, bool H>
 void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr int NT = TP * (M / E);
   const int nt = x_tiles(g, TP);
   const unsigned grid = (unsigned)(TP == 16 ? nt : (nt + 15) / 16 * 16);
   if (kind == KX_FWD)
-    hipLaunchKernelGGL((k_x_fwd<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+    hipLaunchKernelGGL((k_x_fwd<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
   else if (kind == KX_MID)
-    hipLaunchKernelGGL((k_x_mid<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+    hipLaunchKernelGGL((k_x_mid<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
   else if (kind == KX_LAST_FWD)
-    hipLaunchKernelGGL((k_x_last_fwd<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last_fwd<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
   else
-    hipLaunchKernelGGL((k_x_last<M, E, TP>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
 }
 
+// H: the image fills exactly half the x transform (ix == M, a same-size
+// kernel): the row bounds are known at compile time.
 template <int M, int E>
-void launch_x_e(int tp, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  if (tp == 8)
-    launch_x_t<M, E, 8>(kind, g, b, s);
-  else
-    launch_x_t<M, E, 16>(kind, g, b, s);
+void launch_x_e(int tp, bool h, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  if (tp == 8) {
+    if (h) launch_x_t<M, E, 8, true>(kind, g, b, s);
+    else launch_x_t<M, E, 8, false>(kind, g, b, s);
+  } else {
+    if (h) launch_x_t<M, E, 16, true>(kind, g, b, s);
+    else launch_x_t<M, E, 16, false>(kind, g, b, s);
+  }
 }
 
 template <int M>
-void launch_x(int e, int tp, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_x(int e, int tp, bool h, Kind kind, const Geom &g, const Bufs &b,
+              hipStream_t s) {
   if constexpr (M % 3 == 0) {
-    launch_x_e<M, 12>(tp, kind, g, b, s);
+    launch_x_e<M, 12>(tp, h, kind, g, b, s);
   } else if constexpr (M == 64) {
-    launch_x_e<M, 8>(tp, kind, g, b, s);
+    launch_x_e<M, 8>(tp, h, kind, g, b, s);
   } else {
     if (e == 16)
-      launch_x_e<M, 16>(tp, kind, g, b, s);
+      launch_x_e<M, 16>(tp, h, kind, g, b, s);
     else
-      launch_x_e<M, 8>(tp, kind, g, b, s);
+      launch_x_e<M, 8>(tp, h, kind, g, b, s);
   }
 }
 
 void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1 : 2;
-  const int e = tu.xe[j], tp = tu.xt[j];
+  const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1 : (kind == KX_LAST) ? 2 : 3;
+  const int jt = j < 3 ? j : 2;  // x_last_fwd takes x_last's launch shape
+  const int e = tu.xe[jt], tp = tu.xt[jt];
+  const bool h = ((tu.xh >> j) & 1) && 2 * g.ix == g.nx;
   switch (g.nx / 2) {
-    case 64: launch_x<64>(e, tp, kind, g, b, s); break;
-    case 96: launch_x<96>(e, tp, kind, g, b, s); break;
-    case 128: launch_x<128>(e, tp, kind, g, b, s); break;
-    case 192: launch_x<192>(e, tp, kind, g, b, s); break;
-    case 256: launch_x<256>(e, tp, kind, g, b, s); break;
-    case 384: launch_x<384>(e, tp, kind, g, b, s); break;
-    case 512: launch_x<512>(e, tp, kind, g, b, s); break;
+    case 64: launch_x<64>(e, tp, h, kind, g, b, s); break;
+    case 96: launch_x<96>(e, tp, h, kind, g, b, s); break;
+    case 128: launch_x<128>(e, tp, h, kind, g, b, s); break;
+    case 192: launch_x<192>(e, tp, h, kind, g, b, s); break;
+    case 256: launch_x<256>(e, tp, h, kind, g, b, s); break;
+    case 384: launch_x<384>(e, tp, h, kind, g, b, s); break;
+    case 512: launch_x<512>(e, tp, h, kind, g, b, s); break;
   }
 }
 
@@ -1064,6 +1102,59 @@ void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipSt
   }
 }
 
+// Forward FFT along z of every (kx, ky) column over all nz planes, in place
+// (the last pass of the OTF); 16 columns per workgroup.
+template <int N, int E>
+__global__ void __launch_bounds__(kTPz *(N / E))
+    k_zf(float2 *__restrict__ A, const float2 *__restrict__ gtw, Geom g) {
+  constexpr int TP = kTPz;
+  constexpr int T = N / E;
+  constexpr int NT = TP * T;
+  __shared__ float2 lds[TP * N];
+  __shared__ float2 tw[N];
+  const int q = threadIdx.x % TP, t = threadIdx.x / TP;
+  const int ntile = g.ny / TP;
+  const int kx = blockIdx.x / ntile;
+  const int y0 = (blockIdx.x % ntile) * TP;
+  const long long step = (long long)T * g.sz;
+  float2 *const a0 = A + (long long)t * g.sz + (long long)kx * g.ny + y0 + q;
+  TableRegs<NT, N> twr;
+  twr.load(gtw);
+  float2 v[E];
+  {
+    const float2 *p = a0;
+#pragma unroll
+    for (int e = 0; e < E; e++, p += step) v[e] = *p;
+  }
+  twr.store(tw);
+  __syncthreads();
+  const InterLds<TP> L{lds + q};
+  fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
+  {
+    float2 *p = a0;
+#pragma unroll
+    for (int e = 0; e < E; e++, p += step) *p = v[e];
+  }
+}
+
+template <int N>
+void launch_zf(const Geom &g, float2 *A, const float2 *tw, hipStream_t s) {
+  constexpr int E = (N % 3 == 0) ? 12 : (N == 64) ? 8 : 16;
+  hipLaunchKernelGGL((k_zf<N, E>), dim3((unsigned)(g.hx * (g.ny / kTPz))),
+                     dim3(kTPz * (N / E)), 0, s, A, tw, g);
+}
+
+void dispatch_zf(const Geom &g, float2 *A, const float2 *tw, hipStream_t s) {
+  switch (g.nz) {
+    case 64: launch_zf<64>(g, A, tw, s); break;
+    case 96: launch_zf<96>(g, A, tw, s); break;
+    case 128: launch_zf<128>(g, A, tw, s); break;
+    case 192: launch_zf<192>(g, A, tw, s); break;
+    case 256: launch_zf<256>(g, A, tw, s); break;
+    case 512: launch_zf<512>(g, A, tw, s); break;
+  }
+}
+
 // Default shapes, then the environment override.
 Tune pick_tune(const Geom &g) {
   Tune t;
@@ -1090,6 +1181,7 @@ Tune pick_tune(const Geom &g) {
       else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
       else if (std::sscanf(p, "zt=%d", &v) == 1) t.zt = v;
       else if (std::sscanf(p, "fx=%d", &v) == 1) t.fx = v;
+      else if (std::sscanf(p, "xh=%d", &v) == 1) t.xh = v;
       while (*p && *p != ',') p++;
       if (*p == ',') p++;
     }
@@ -1152,26 +1244,19 @@ class FastRL : public RLEngine {
       CL_TRY(pad.alloc(sizeof(float) * w_.V));
       const int kshift[3] = {-w_.kd.x / 2, -w_.kd.y / 2, -w_.kd.z / 2};
       CL_TRY(k_pad_shift(pad.f(), w_.wd, psf_d, w_.kd, kshift, 0.f, s));
-      AxisPlan px, py, pz;
-      CL_TRY(axis_plan(g_.nx, &px));
-      CL_TRY(axis_plan(g_.ny, &py));
-      CL_TRY(axis_plan(g_.nz, &pz));
-      PencilGeom gx;  // rows (z, y) -> element kx at stride ny
-      gx.nA = g_.nz; gx.nB = g_.ny;
-      gx.ia = (long long)g_.ny * g_.nx; gx.ib = g_.nx; gx.ie = 1;
-      gx.oa = g_.sz; gx.ob = 1; gx.oe = g_.ny;
-      gx.Lin = g_.nx; gx.Lout = g_.hx;
-      CL_TRY(fft_r2c_rows(pad.f(), otf_.c(), px, gx, s));
-      PencilGeom gy;  // pencils (z, kx) contiguous along y
-      gy.nA = g_.nz; gy.nB = g_.hx;
-      gy.ia = gy.oa = g_.sz; gy.ib = gy.ob = g_.ny; gy.ie = gy.oe = 1;
-      gy.Lin = gy.Lout = g_.ny;
-      CL_TRY(fft_c2c_pass(otf_.c(), otf_.c(), py, gy, -1, s));
-      PencilGeom gz;  // pencils (kx, ky) along z
-      gz.nA = 1; gz.nB = g_.sz;
-      gz.ia = gz.oa = 0; gz.ib = gz.ob = 1; gz.ie = gz.oe = g_.sz;
-      gz.Lin = gz.Lout = g_.nz;
-      CL_TRY(fft_c2c_pass(otf_.c(), otf_.c(), pz, gz, -1, s));
+      // the iteration's own x and y kernels over the whole work domain (the
+      // "image" of this transform is every row), then a forward z pass
+      Geom go = g_;
+      go.ix = g_.nx;
+      go.iy = g_.ny;
+      go.iz = g_.nz;
+      Bufs bo = b_;
+      bo.u = pad.f();
+      bo.S = otf_.c();
+      dispatch_x(tu_, KX_FWD, go, bo, s);
+      dispatch_y(tu_, true, go, bo, s);
+      dispatch_zf(go, otf_.c(), b_.twZ, s);
+      CL_TRY(launch_check());
       CL_HIP(hipStreamSynchronize(s));  // pad is released on return
     }
     // E_obs: the padded observation's sum (zeros outside the corner)
