@@ -91,6 +91,8 @@ _SIGS = [
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
       ctypes.POINTER(ctypes.c_int), ctypes.c_int]),
     ("ClarityEx_RLPlanSetTiming", _R, [ctypes.c_void_p, ctypes.c_int]),
+    ("ClarityEx_RLPlanAutotune", _R, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                      ctypes.c_int]),
     ("ClarityEx_RLPlanTimings", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
       ctypes.c_int, ctypes.c_int]),
@@ -373,6 +375,15 @@ class RLPlan:
     def set_timing(self, on: bool):
         _check(lib().ClarityEx_RLPlanSetTiming(self.h, 1 if on else 0),
                "ClarityEx_RLPlanSetTiming")
+
+    def autotune(self, iterations: int = 3) -> str:
+        """Times launch-shape candidates on this plan's inputs and keeps the
+        fastest (ClarityEx_RLPlanAutotune); returns the choice in
+        CLARITY_FAST_TUNE syntax ('' for the generic engine)."""
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().ClarityEx_RLPlanAutotune(self.h, iterations, buf, 256),
+               "ClarityEx_RLPlanAutotune")
+        return buf.value.decode()
 
     def timings(self, reset: bool = False):
         cap = 32

@@ -1191,6 +1191,14 @@ Tune pick_tune(const Geom &g) {
 
 // --------------------------------------------------------------- engine
 
+// Launch shapes chosen by autotuning, per work shape, for the process
+// (accessed under the library lock).
+std::map<std::vector<int>, Tune> &tune_cache() {
+  static std::map<std::vector<int>, Tune> cache;
+  return cache;
+}
+std::vector<int> tune_key(const Geom &g) { return {g.ix, g.iy, g.iz, g.nx, g.ny, g.nz}; }
+
 class FastRL : public RLEngine {
  public:
   explicit FastRL(const Work &geom) {
@@ -1200,6 +1208,10 @@ class FastRL : public RLEngine {
     g_.hx = g_.nx / 2 + 1;
     g_.sz = (long long)g_.hx * g_.ny;
     tu_ = pick_tune(g_);
+    // a shape autotuned earlier in this process keeps its choice, unless
+    // the environment fixes the shapes
+    auto hit = tune_cache().find(tune_key(g_));
+    if (hit != tune_cache().end() && !std::getenv("CLARITY_FAST_TUNE")) tu_ = hit->second;
     nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
   }
   ~FastRL() override {
@@ -1221,7 +1233,9 @@ class FastRL : public RLEngine {
     CL_TRY(u_.alloc(sizeof(float) * w_.IV));
     CL_TRY(S_.alloc(sizeof(float2) * (size_t)g_.iz * g_.sz));
     CL_TRY(otf_.alloc(sizeof(float2) * (size_t)g_.nz * g_.sz));
-    CL_TRY(parts_.alloc(sizeof(double) * (size_t)(nparts_ > kParts ? nparts_ : kParts)));
+    // sized for the most x tiles any launch shape makes (8-row tiles)
+    const int most = x_tiles(g_, 8);
+    CL_TRY(parts_.alloc(sizeof(double) * (size_t)(most > kParts ? most : kParts)));
     CL_TRY(scal_.alloc(sizeof(float) * 4));
     AxisPlan pm, pn, py, pz;
     CL_TRY(axis_plan(g_.nx / 2, &pm));
@@ -1376,6 +1390,43 @@ class FastRL : public RLEngine {
     return CLARITY_SUCCESS;
   }
 
+  // Coordinate descent over the launch-shape knobs, each candidate timed
+  // with HIP events over `iterations` iterations (after one warm-up
+  // iteration) on the plan's own inputs; a candidate replaces the current
+  // shape only when it is at least 1 % faster.  The estimate and its scale
+  // are saved first and restored after, so tuning leaves no trace in the
+  // results.  The choice is cached per work shape for the process and used
+  // by later plans of that shape.
+  ClarityResult_t autotune(int iterations, char *desc, int cap, hipStream_t s) override {
+    if (iterations < 1) return CLARITY_INVALID_ARGUMENT;
+    DevBuf su, ss;
+    CL_TRY(su.alloc(sizeof(float) * w_.IV));
+    CL_TRY(ss.alloc(sizeof(float) * 4));
+    CL_HIP(hipMemcpyAsync(su.p, u_.p, sizeof(float) * w_.IV, hipMemcpyDeviceToDevice, s));
+    CL_HIP(hipMemcpyAsync(ss.p, scal_.p, sizeof(float) * 4, hipMemcpyDeviceToDevice, s));
+    const bool timing = timing_;
+    timing_ = false;
+    const bool graph = use_graph_;
+    use_graph_ = false;  // time plain launches; graphs are rebuilt after
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ClarityResult_t r = from_hip(hipEventCreate(&e0));
+    if (r == CLARITY_SUCCESS) r = from_hip(hipEventCreate(&e1));
+    if (r == CLARITY_SUCCESS) r = tune_search(iterations, e0, e1, s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    use_graph_ = graph;
+    timing_ = timing;
+    CL_HIP(hipMemcpyAsync(u_.p, su.p, sizeof(float) * w_.IV, hipMemcpyDeviceToDevice, s));
+    CL_HIP(hipMemcpyAsync(scal_.p, ss.p, sizeof(float) * 4, hipMemcpyDeviceToDevice, s));
+    CL_HIP(hipStreamSynchronize(s));
+    if (desc && cap > 0)
+      std::snprintf(desc, (size_t)cap,
+                    "fe=%d,ft=%d,me=%d,mt=%d,le=%d,lt=%d,yp=%d,ze=%d,zt=%d,fx=%d,xh=%d",
+                    tu_.xe[0], tu_.xt[0], tu_.xe[1], tu_.xt[1], tu_.xe[2], tu_.xt[2], tu_.yp,
+                    tu_.ze, tu_.zt, tu_.fx, tu_.xh);
+    return r;
+  }
+
   int timings(int *count, double *ms, int cap, bool reset) override {
     Runtime &r = rt();
     if (r.stream) (void)hipStreamSynchronize(r.stream);
@@ -1404,6 +1455,78 @@ class FastRL : public RLEngine {
   }
 
  private:
+  // Makes `t` the launch shape: captured graphs describe the old one.
+  void apply_tune(const Tune &t) {
+    tu_ = t;
+    nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
+    for (auto &ge : graph_exec_) {
+      if (ge) (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
+  }
+
+  // Milliseconds per iteration of shape `t` (estimate reset to the
+  // observation, one warm-up iteration, then `iterations` timed).
+  ClarityResult_t time_tune(const Tune &t, int iterations, hipEvent_t e0, hipEvent_t e1,
+                            hipStream_t s, float *ms) {
+    apply_tune(t);
+    CL_TRY(reset(nullptr, s));
+    CL_TRY(iterate(1, s));
+    CL_HIP(hipEventRecord(e0, s));
+    CL_TRY(iterate(iterations, s));
+    CL_HIP(hipEventRecord(e1, s));
+    CL_HIP(hipEventSynchronize(e1));
+    CL_HIP(hipEventElapsedTime(ms, e0, e1));
+    *ms /= (float)iterations;
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t tune_search(int iterations, hipEvent_t e0, hipEvent_t e1, hipStream_t s) {
+    auto &cache = tune_cache();
+    const std::vector<int> key = tune_key(g_);
+    auto hit = cache.find(key);
+    if (hit != cache.end()) {
+      apply_tune(hit->second);
+      return CLARITY_SUCCESS;
+    }
+    Tune best = tu_;
+    float best_ms = 0.f;
+    CL_TRY(time_tune(best, iterations, e0, e1, s, &best_ms));
+    // each knob and the values it may take; invalid combinations fall back
+    // to valid launch shapes inside the dispatchers
+    struct Knob {
+      int *(*field)(Tune &);
+      std::vector<int> values;
+    };
+    const std::vector<Knob> knobs = {
+        {[](Tune &t) { return &t.xt[0]; }, {8, 16}},
+        {[](Tune &t) { return &t.xt[1]; }, {8, 16}},
+        {[](Tune &t) { return &t.xt[2]; }, {8, 16}},
+        {[](Tune &t) { return &t.xe[1]; }, {8, 16}},
+        {[](Tune &t) { return &t.yp; }, {2, 4, 8}},
+        {[](Tune &t) { return &t.ze; }, {8, 16}},
+        {[](Tune &t) { return &t.zt; }, {16, 32, 64}},
+        {[](Tune &t) { return &t.fx; }, {0, 1}},
+        {[](Tune &t) { return &t.xh; }, {0, 7, 15}},
+    };
+    for (const Knob &k : knobs) {
+      for (int v : k.values) {
+        Tune c = best;
+        if (*k.field(c) == v) continue;
+        *k.field(c) = v;
+        float ms = 0.f;
+        CL_TRY(time_tune(c, iterations, e0, e1, s, &ms));
+        if (ms < 0.99f * best_ms) {
+          best = c;
+          best_ms = ms;
+        }
+      }
+    }
+    apply_tune(best);
+    cache[key] = best;
+    return CLARITY_SUCCESS;
+  }
+
   ClarityResult_t run(Kind kind, hipStream_t s) {
     int slot = -1;
     if (timing_) {

@@ -345,6 +345,14 @@ ClarityResult_t ClarityEx_RLPlanSetTiming(ClarityEx_RLPlan plan, int enable) {
   return plan->eng->set_timing(enable != 0);
 }
 
+ClarityResult_t ClarityEx_RLPlanAutotune(ClarityEx_RLPlan plan, int iterations,
+                                         char *description, int cap) {
+  API_LOCK;
+  if (!plan || iterations < 1 || (description && cap < 0)) return CLARITY_INVALID_ARGUMENT;
+  if (!plan->have_inputs) return CLARITY_INVALID_OPERATION;
+  return plan->eng->autotune(iterations, description, cap, rt().stream);
+}
+
 int ClarityEx_RLPlanTimings(ClarityEx_RLPlan plan, int *count, double *ms,
                             int cap, int reset) {
   API_LOCK;

@@ -37,6 +37,14 @@ class RLEngine {
     (void)count; (void)ms; (void)cap; (void)reset;
     return 0;
   }
+  // Picks launch shapes by timing candidates on the current inputs (the
+  // estimate is saved and restored); writes a description of the choice to
+  // `desc` (cap bytes).  Engines without tunable shapes do nothing.
+  virtual ClarityResult_t autotune(int iterations, char *desc, int cap, hipStream_t s) {
+    (void)iterations; (void)s;
+    if (desc && cap > 0) desc[0] = 0;
+    return CLARITY_SUCCESS;
+  }
 
  protected:
   Work w_;

@@ -110,6 +110,17 @@ C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanSetTiming(ClarityEx_RLPlan plan,
 C_FUNC_DEF int ClarityEx_RLPlanTimings(ClarityEx_RLPlan plan, int *count,
                                        double *ms, int cap, int reset);
 
+/* Chooses the fused engine's launch shapes (tile sizes, elements per thread,
+ * pencils per workgroup, chaining, specialisations) for this plan's shape by
+ * timing candidates on its inputs, `iterations` iterations each (coordinate
+ * descent from the built-in defaults).  The estimate is saved and restored,
+ * so results are unchanged; later plans of the same shape in the process use
+ * the choice (unless CLARITY_FAST_TUNE is set).  Writes the chosen shapes, in
+ * CLARITY_FAST_TUNE syntax, to `description` (NUL-terminated, at most `cap`
+ * bytes; may be NULL).  No counterpart in the reference.  Requires inputs. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanAutotune(ClarityEx_RLPlan plan, int iterations,
+                                                    char *description, int cap);
+
 /* Name of the engine running the plan: "fast" (fused power-of-two
  * pipeline) or "generic". */
 C_FUNC_DEF const char *ClarityEx_RLPlanEngine(ClarityEx_RLPlan plan);

@@ -155,12 +155,21 @@ def test_fast_timing_api(clarity_gpu):
     plan.close()
 
 
-def test_fused_x_boundary_matches_split(clarity_gpu):
+@pytest.mark.parametrize("tune,exact", [("xh=0", True), (None, False)])
+def test_fused_x_boundary_matches_split(clarity_gpu, tune, exact):
     """Iterations chained through x_last_fwd equal the same iterations run
-    one call at a time (x_last, then x_fwd), bit for bit."""
+    one call at a time (x_last, then x_fwd).  Without the same-size x
+    specialisations (xh=0) both routes run the same loads and arithmetic, so
+    bit for bit; with them (the default) x_fwd and x_last may contract the
+    butterflies of the known-zero half differently, so to float rounding."""
     api = clarity_gpu
     case = synth.make_config(1)
-    plan = api.RLPlan(case.obs.shape, case.psf.shape)
+    if tune is not None:
+        os.environ["CLARITY_FAST_TUNE"] = tune
+    try:
+        plan = api.RLPlan(case.obs.shape, case.psf.shape)
+    finally:
+        os.environ.pop("CLARITY_FAST_TUNE", None)
     plan.set_inputs(case.obs, case.psf)
     plan.reset()
     plan.iterate(5)
@@ -170,7 +179,10 @@ def test_fused_x_boundary_matches_split(clarity_gpu):
         plan.iterate(1)
     split = plan.estimate()
     plan.close()
-    assert np.array_equal(chained, split)
+    if exact:
+        assert np.array_equal(chained, split)
+    else:
+        assert rel_l2(chained, split) <= 1e-6
 
 
 def test_batch_uneven_volumes(clarity_gpu):
@@ -214,3 +226,32 @@ def test_graph_replay_matches_eager(clarity_gpu):
         assert eager.scale() == graph.scale()
     eager.close()
     graph.close()
+
+
+def test_autotune_leaves_results_unchanged(clarity_gpu):
+    """Autotuning times launch shapes on the plan's own inputs; the estimate
+    it found is restored bit for bit, and iterating with the chosen shapes
+    gives the same RL result to float rounding."""
+    api = clarity_gpu
+    # a shape no other test uses: the choice is cached for the process
+    obs, psf = _case((24, 64, 64), (40, 64, 64), 5)
+    plan = api.RLPlan(obs.shape, psf.shape)
+    assert plan.engine == "fast"
+    plan.set_inputs(obs, psf)
+    plan.reset()
+    plan.iterate(4)
+    before = plan.estimate()
+    choice = plan.autotune(2)
+    print(f"[fast] autotune image 64x64x24 kernel 64x64x40: {choice}", flush=True)
+    assert "zt=" in choice and "fx=" in choice
+    assert np.array_equal(plan.estimate(), before)
+    plan.reset()
+    plan.iterate(4)
+    assert rel_l2(plan.estimate(), before) <= 1e-5
+    plan.close()
+    # a new plan of the same shape starts from the tuned shapes
+    again = api.RLPlan(obs.shape, psf.shape)
+    again.set_inputs(obs, psf)
+    again.iterate(4)
+    assert rel_l2(again.estimate(), before) <= 1e-5
+    again.close()
