@@ -10,7 +10,7 @@
 //
 //   x_fwd   u (image rows)            -> S      R2C along x, rows y<iy, z<iz
 //   y_fwd   S (y<iy)                  -> S      FFT along y, all ny outputs
-//   z_mod   S (z<iz), OTF             -> S      FFT z, * OTF/N, IFFT z, z<iz kept
+//   z_mod   S (z<iz), K               -> S      FFT z of S and K, * /N, IFFT z, z<iz kept
 //   y_inv   S                         -> S      IFFT along y, y<iy kept
 //   x_mid   S, obs                    -> S      C2R x, r = guard(obs/(c*s1)), R2C x
 //   y_fwd, z_mod, y_inv               (second convolution)
@@ -22,7 +22,12 @@
 // the next x_fwd run as one kernel (x_last_fwd: u' is transformed while
 // still in registers), so a run of n iterations launches x_fwd and x_last
 // once and x_last_fwd n - 1 times.  The spectrum S has layout (z, kx, y), y
-// fastest, z < iz only; the OTF has layout (kz, kx, ky).  Every transform is
+// fastest, z < iz only.  The kernel enters as K, its x and y transform
+// before the z transform (the OTF's last step), laid out (p, kx, ky) over
+// only the kz planes p the -k/2 shifted kernel occupies: z_mod finishes each
+// OTF column along z in registers instead of reading a stored OTF of all nz
+// planes (a quarter of its bytes at a same-size kernel, for one more FFT).
+// Every transform is
 // a register-resident Stockham FFT: a thread holds E elements of its pencil
 // (element t + T*e in v[e]), butterflies of radix E run in registers and
 // the stages exchange through LDS.  The x passes use the half-length complex
@@ -32,6 +37,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace clarity {
@@ -303,7 +309,11 @@ struct Geom {
   int ix, iy, iz;  // image
   int nx, ny, nz;  // work
   int hx;          // nx/2 + 1
-  long long sz;    // S / OTF stride of z: hx*ny
+  long long sz;    // S / K stride of z: hx*ny
+  // kernel planes: the -kz/2 shifted kernel occupies work planes z < ka
+  // and z >= nz - (kz - ka), stored in K as planes p = z and
+  // p = z - (nz - kz) (ka = kz - kz/2)
+  int kz, ka;
 };
 
 // Copies a twiddle table into LDS (cooperatively; caller synchronises).
@@ -825,13 +835,19 @@ __global__ void __launch_bounds__(PPW *(N / E))
 }
 
 // A workgroup transforms TP adjacent ky columns of one kx (TP * 8 bytes
-// contiguous per z plane); TP lanes of a wavefront read one plane.  H: the
-// image is half the work domain along z (iz = N/2), so the planes read and
-// written are exactly elements e < E/2: no bounds tests.
+// contiguous per z plane); TP lanes of a wavefront read one plane.  Each
+// column of S is multiplied by the matching OTF column, the z transform of
+// the kernel column of K, made here in registers (the same FFT as the
+// spectrum's; the kernel, shifted by -kz/2, has its kz nonzero planes at
+// z < ka and z >= nz - (kz - ka)).  H: image and kernel are both half the
+// work domain along z (iz = kz = N/2), so the planes of S read and written
+// are exactly elements e < E/2, and the kernel's are e < E/4 (planes
+// z < N/4) and e >= 3E/4 (planes z >= 3N/4): no bounds tests.
 template <int N, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(N / E))
-    k_z(float2 *__restrict__ S, const float2 *__restrict__ otf,
+    k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
         const float2 *__restrict__ gtw, Geom g, float scale) {
+  static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
   constexpr int T = N / E;
   constexpr int NT = TP * T;
   __shared__ float2 lds[TP * N];
@@ -844,12 +860,10 @@ __global__ void __launch_bounds__(TP *(N / E))
   // element (t + T*e) of the column: one 64-bit step of T planes per e
   const long long step = (long long)T * g.sz;
   float2 *const s0 = S + (long long)t * g.sz + col;
-  const float2 *const h0 = otf + (long long)t * g.sz + col;
-  // With 32 or more columns the tile's LDS already limits a CU to 16
-  // wavefronts, so registers are free up to 128: the OTF column is then
-  // loaded up front, behind the twiddles and the spectrum (the counter of
-  // outstanding loads is in order), and arrives during the forward FFT.
-  constexpr bool PRE = TP >= 32;
+  const float2 *const k0 = K + (long long)t * g.sz + col;
+  // Every load up front, in the order of use (the counter of outstanding
+  // loads is in order): twiddles, the spectrum column, the kernel column,
+  // which arrives during the spectrum's forward FFT.
   TableRegs<NT, N> twr;
   twr.load(gtw);
   float2 v[E];
@@ -863,23 +877,40 @@ __global__ void __launch_bounds__(TP *(N / E))
         v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
     }
   }
-  float2 h[PRE ? E : 1];
-  if constexpr (PRE) {
+  float2 h[E];
 #pragma unroll
-    for (int e = 0; e < E; e++) h[e] = h0[e * step];
+  for (int e = 0; e < E; e++) {
+    if constexpr (H) {
+      // plane z = t + T*e; upper planes sit N/2 = (E/2)*T planes lower in K
+      if (e < E / 4)
+        h[e] = k0[e * step];
+      else if (e >= 3 * E / 4)
+        h[e] = k0[(e - E / 2) * step];
+      else
+        h[e] = make_float2(0.f, 0.f);
+    } else {
+      // conditional loads: planes between the two ranges take no register
+      // until the FFT (unconditional loads with a 0/1 factor need ~20 more
+      // VGPRs here and spill at 1024 threads)
+      const int z = t + T * e;
+      const int hi = g.nz - (g.kz - g.ka);  // first upper plane
+      if (z < g.ka)
+        h[e] = k0[e * step];
+      else if (z >= hi)
+        h[e] = K[(long long)(z - (g.nz - g.kz)) * g.sz + col];
+      else
+        h[e] = make_float2(0.f, 0.f);
+    }
   }
   twr.store(tw);
   __syncthreads();
   const InterLds<TP> L{lds + q};
   fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
+  fft_stages<N, E, 1, -1>(h, t, L, tw, typename Plan<N, E>::Rad{});
   // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    float2 hv;
-    if constexpr (PRE)
-      hv = h[e];
-    else
-      hv = h0[e * step];
+    const float2 hv = h[e];
     const float2 a = v[e];
     v[e] = make_float2(scale * (a.x * hv.x - a.y * hv.y), scale * (a.x * hv.y + a.y * hv.x));
   }
@@ -906,9 +937,10 @@ const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod",  "y_inv",
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
 // Launch shape of the fused kernels; defaults per size, overridable with
-// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zt=16,fx=1"
+// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zt=16,fx=1,xh=7"
 // (elements per thread and tile rows of x_fwd, x_mid, x_last; y pencils per
-// workgroup; z elements per thread and tile columns; x_last_fwd chaining).
+// workgroup; z elements per thread and tile columns; x_last_fwd chaining;
+// same-size specialisations); any subset, the rest keep their defaults.
 struct Tune {
   int xe[3] = {0, 0, 0};     // x_fwd, x_mid, x_last: elements per thread
   int xt[3] = {16, 16, 16};  // rows per tile
@@ -916,7 +948,8 @@ struct Tune {
   int fx = 1;                // chain x_last into the next x_fwd
   int xh = 7;                // bit j: same-size specialisation of x kernel j
                              // (x_fwd, x_mid, x_last, x_last_fwd); measured
-                             // slower for x_last_fwd at 512x512x128
+                             // slower for x_last_fwd with 8-row tiles and 8
+                             // elements per thread
 };
 
 bool x_supported(int M) {
@@ -932,7 +965,7 @@ bool z_supported(int N) {
 
 struct Bufs {
   float *u, *obs, *scal;  // scal: E, S, c
-  float2 *S, *otf;
+  float2 *S, *K;          // spectrum; kernel after its x and y transforms
   double *parts;
   const float2 *twM, *twN, *twY, *twZ;
 };
@@ -1054,12 +1087,12 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
 template <int N, int E, int TP>
 void launch_z_t(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
   const unsigned grid = (unsigned)(g.hx * (g.ny / TP));
-  if (2 * g.iz == N)
+  if (2 * g.iz == N && 2 * g.kz == N)
     hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.otf, b.twZ, g, scale);
+                       b.K, b.twZ, g, scale);
   else
     hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.otf, b.twZ, g, scale);
+                       b.K, b.twZ, g, scale);
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
@@ -1102,59 +1135,6 @@ void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipSt
   }
 }
 
-// Forward FFT along z of every (kx, ky) column over all nz planes, in place
-// (the last pass of the OTF); 16 columns per workgroup.
-template <int N, int E>
-__global__ void __launch_bounds__(kTPz *(N / E))
-    k_zf(float2 *__restrict__ A, const float2 *__restrict__ gtw, Geom g) {
-  constexpr int TP = kTPz;
-  constexpr int T = N / E;
-  constexpr int NT = TP * T;
-  __shared__ float2 lds[TP * N];
-  __shared__ float2 tw[N];
-  const int q = threadIdx.x % TP, t = threadIdx.x / TP;
-  const int ntile = g.ny / TP;
-  const int kx = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
-  const long long step = (long long)T * g.sz;
-  float2 *const a0 = A + (long long)t * g.sz + (long long)kx * g.ny + y0 + q;
-  TableRegs<NT, N> twr;
-  twr.load(gtw);
-  float2 v[E];
-  {
-    const float2 *p = a0;
-#pragma unroll
-    for (int e = 0; e < E; e++, p += step) v[e] = *p;
-  }
-  twr.store(tw);
-  __syncthreads();
-  const InterLds<TP> L{lds + q};
-  fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
-  {
-    float2 *p = a0;
-#pragma unroll
-    for (int e = 0; e < E; e++, p += step) *p = v[e];
-  }
-}
-
-template <int N>
-void launch_zf(const Geom &g, float2 *A, const float2 *tw, hipStream_t s) {
-  constexpr int E = (N % 3 == 0) ? 12 : (N == 64) ? 8 : 16;
-  hipLaunchKernelGGL((k_zf<N, E>), dim3((unsigned)(g.hx * (g.ny / kTPz))),
-                     dim3(kTPz * (N / E)), 0, s, A, tw, g);
-}
-
-void dispatch_zf(const Geom &g, float2 *A, const float2 *tw, hipStream_t s) {
-  switch (g.nz) {
-    case 64: launch_zf<64>(g, A, tw, s); break;
-    case 96: launch_zf<96>(g, A, tw, s); break;
-    case 128: launch_zf<128>(g, A, tw, s); break;
-    case 192: launch_zf<192>(g, A, tw, s); break;
-    case 256: launch_zf<256>(g, A, tw, s); break;
-    case 512: launch_zf<512>(g, A, tw, s); break;
-  }
-}
-
 // Default shapes, then the environment override.
 Tune pick_tune(const Geom &g) {
   Tune t;
@@ -1163,10 +1143,25 @@ Tune pick_tune(const Geom &g) {
     t.xe[j] = xe;
     t.xt[j] = 16;
   }
-  if (g.nx / 2 == 512) t.xt[0] = t.xt[1] = t.xt[2] = 8;  // measured at 512x512x128
+  if (g.nx / 2 == 512) {
+    t.xt[0] = t.xt[1] = t.xt[2] = 8;
+    // Measured at 512x512x128: 16-row tiles read 128-byte spans of every
+    // spectrum row (4.8 TB/s for the access pattern alone, 3.9 TB/s with 8
+    // rows).  With 16 elements per thread the same-size x_mid and
+    // x_last_fwd stay at <= 124 VGPRs, two 512-thread workgroups per CU;
+    // without the specialisation they need ~140 and lose half of that.
+    if (2 * g.ix == g.nx) {
+      t.xe[1] = t.xe[2] = 16;
+      t.xt[1] = t.xt[2] = 16;
+      t.xh = 15;
+    }
+  }
   t.yp = 4;
   t.ze = (g.nz == 128) ? 8 : 16;
   t.zt = (g.nz >= 256) ? 32 : 16;  // measured: 256-byte plane segments at nz 256
+  // at nz 512 the general z kernel (both columns' bounds tested) spills
+  // with 32 columns (1024 threads, so at most 128 VGPRs); 16 do not
+  if (g.nz == 512 && !(2 * g.iz == g.nz && 2 * g.kz == g.nz)) t.zt = 16;
   if (const char *s = std::getenv("CLARITY_FAST_TUNE")) {
     int v;
     const char *p = s;
@@ -1197,7 +1192,9 @@ std::map<std::vector<int>, Tune> &tune_cache() {
   static std::map<std::vector<int>, Tune> cache;
   return cache;
 }
-std::vector<int> tune_key(const Geom &g) { return {g.ix, g.iy, g.iz, g.nx, g.ny, g.nz}; }
+std::vector<int> tune_key(const Geom &g) {
+  return {g.ix, g.iy, g.iz, g.nx, g.ny, g.nz, g.kz};
+}
 
 class FastRL : public RLEngine {
  public:
@@ -1207,6 +1204,8 @@ class FastRL : public RLEngine {
     g_.nx = geom.wd.x; g_.ny = geom.wd.y; g_.nz = geom.wd.z;
     g_.hx = g_.nx / 2 + 1;
     g_.sz = (long long)g_.hx * g_.ny;
+    g_.kz = geom.kd.z;
+    g_.ka = geom.kd.z - geom.kd.z / 2;
     tu_ = pick_tune(g_);
     // a shape autotuned earlier in this process keeps its choice, unless
     // the environment fixes the shapes
@@ -1232,7 +1231,7 @@ class FastRL : public RLEngine {
     CL_TRY(obs_.alloc(sizeof(float) * w_.IV));
     CL_TRY(u_.alloc(sizeof(float) * w_.IV));
     CL_TRY(S_.alloc(sizeof(float2) * (size_t)g_.iz * g_.sz));
-    CL_TRY(otf_.alloc(sizeof(float2) * (size_t)g_.nz * g_.sz));
+    CL_TRY(K_.alloc(sizeof(float2) * (size_t)g_.kz * g_.sz));
     // sized for the most x tiles any launch shape makes (8-row tiles)
     const int most = x_tiles(g_, 8);
     CL_TRY(parts_.alloc(sizeof(double) * (size_t)(most > kParts ? most : kParts)));
@@ -1244,7 +1243,7 @@ class FastRL : public RLEngine {
     CL_TRY(axis_plan(g_.nz, &pz));
     b_.twM = pm.tw; b_.twN = pn.tw; b_.twY = py.tw; b_.twZ = pz.tw;
     b_.u = u_.f(); b_.obs = obs_.f(); b_.scal = scal_.f();
-    b_.S = S_.c(); b_.otf = otf_.c(); b_.parts = parts_.d();
+    b_.S = S_.c(); b_.K = K_.c(); b_.parts = parts_.d();
     return CLARITY_SUCCESS;
   }
 
@@ -1252,24 +1251,28 @@ class FastRL : public RLEngine {
                              hipStream_t s) override {
     CL_HIP(hipMemcpyAsync(obs_.p, obs_d, sizeof(float) * w_.IV,
                           hipMemcpyDeviceToDevice, s));
-    // OTF = R2C of the padded, -k/2 shifted kernel, laid out (kz, kx, ky)
+    // K = the x and y transforms of the padded, -k/2 shifted kernel (the
+    // OTF but for its z transform, which z_mod makes), over only the kz
+    // planes the shifted kernel occupies: shifting by -kz/2 modulo kz
+    // instead of nz puts work planes z < ka at p = z and the planes
+    // z >= nz - kz/2 at p = z - (nz - kz), the layout z_mod reads.
     {
       DevBuf pad;
-      CL_TRY(pad.alloc(sizeof(float) * w_.V));
+      const Dims pd{g_.nx, g_.ny, g_.kz};
+      CL_TRY(pad.alloc(sizeof(float) * (size_t)pd.voxels()));
       const int kshift[3] = {-w_.kd.x / 2, -w_.kd.y / 2, -w_.kd.z / 2};
-      CL_TRY(k_pad_shift(pad.f(), w_.wd, psf_d, w_.kd, kshift, 0.f, s));
-      // the iteration's own x and y kernels over the whole work domain (the
-      // "image" of this transform is every row), then a forward z pass
+      CL_TRY(k_pad_shift(pad.f(), pd, psf_d, w_.kd, kshift, 0.f, s));
+      // the iteration's own x and y kernels over every row of the kz
+      // planes (the "image" of this transform is all of them)
       Geom go = g_;
       go.ix = g_.nx;
       go.iy = g_.ny;
-      go.iz = g_.nz;
+      go.iz = g_.kz;
       Bufs bo = b_;
       bo.u = pad.f();
-      bo.S = otf_.c();
+      bo.S = K_.c();
       dispatch_x(tu_, KX_FWD, go, bo, s);
       dispatch_y(tu_, true, go, bo, s);
-      dispatch_zf(go, otf_.c(), b_.twZ, s);
       CL_TRY(launch_check());
       CL_HIP(hipStreamSynchronize(s));  // pad is released on return
     }
@@ -1362,11 +1365,11 @@ class FastRL : public RLEngine {
     const double IV = (double)w_.IV;
     const double Sp = (double)g_.iz * g_.hx * g_.iy;  // spectrum, y < iy
     const double Sf = (double)g_.iz * g_.hx * g_.ny;  // spectrum, all ky
-    const double O = (double)g_.nz * g_.hx * g_.ny;   // OTF
+    const double Kc = (double)g_.kz * g_.hx * g_.ny;  // kernel planes of K
     const double b[KNUM] = {
         f * IV + c * Sp,                // x_fwd
         c * Sp + c * Sf,                // y_fwd
-        c * Sf + c * O + c * Sf,        // z_mod
+        c * Sf + c * Kc + c * Sf,       // z_mod
         c * Sf + c * Sp,                // y_inv
         c * Sp + f * IV + c * Sp,       // x_mid
         c * Sp + f * IV + f * IV,       // x_last
@@ -1492,35 +1495,49 @@ class FastRL : public RLEngine {
     Tune best = tu_;
     float best_ms = 0.f;
     CL_TRY(time_tune(best, iterations, e0, e1, s, &best_ms));
-    // each knob and the values it may take; invalid combinations fall back
-    // to valid launch shapes inside the dispatchers
+    // Each knob and the values it may take; invalid combinations fall back
+    // to valid launch shapes inside the dispatchers.  The x knobs set
+    // elements per thread and tile rows together (value e * 100 + rows), the
+    // x_last knob also the same-size flag of x_last_fwd (+ 1): these only
+    // pay off jointly (the register count decides the occupancy).
     struct Knob {
-      int *(*field)(Tune &);
+      void (*set)(Tune &, int);
       std::vector<int> values;
     };
     const std::vector<Knob> knobs = {
-        {[](Tune &t) { return &t.xt[0]; }, {8, 16}},
-        {[](Tune &t) { return &t.xt[1]; }, {8, 16}},
-        {[](Tune &t) { return &t.xt[2]; }, {8, 16}},
-        {[](Tune &t) { return &t.xe[1]; }, {8, 16}},
-        {[](Tune &t) { return &t.yp; }, {2, 4, 8}},
-        {[](Tune &t) { return &t.ze; }, {8, 16}},
-        {[](Tune &t) { return &t.zt; }, {16, 32, 64}},
-        {[](Tune &t) { return &t.fx; }, {0, 1}},
-        {[](Tune &t) { return &t.xh; }, {0, 7, 15}},
+        {[](Tune &t, int v) { t.xe[0] = v / 100, t.xt[0] = v % 100; }, {808, 816, 1608, 1616}},
+        {[](Tune &t, int v) { t.xe[1] = v / 100, t.xt[1] = v % 100; }, {808, 816, 1608, 1616}},
+        {[](Tune &t, int v) {
+           t.xe[2] = v / 1000, t.xt[2] = (v / 10) % 100, t.xh = (t.xh & 7) | ((v % 10) << 3);
+         },
+         {8080, 8081, 8160, 8161, 16080, 16081, 16160, 16161}},
+        {[](Tune &t, int v) { t.xh = (t.xh & 8) | v; }, {0, 7}},
+        {[](Tune &t, int v) { t.yp = v; }, {2, 4, 8}},
+        {[](Tune &t, int v) { t.ze = v; }, {8, 16}},
+        {[](Tune &t, int v) { t.zt = v; }, {16, 32, 64}},
+        {[](Tune &t, int v) { t.fx = v; }, {0, 1}},
     };
-    for (const Knob &k : knobs) {
-      for (int v : k.values) {
-        Tune c = best;
-        if (*k.field(c) == v) continue;
-        *k.field(c) = v;
-        float ms = 0.f;
-        CL_TRY(time_tune(c, iterations, e0, e1, s, &ms));
-        if (ms < 0.99f * best_ms) {
-          best = c;
-          best_ms = ms;
+    auto same = [](const Tune &a, const Tune &b) {
+      return std::memcmp(&a, &b, sizeof(Tune)) == 0;
+    };
+    // coordinate descent, a second sweep when the first moved
+    for (int sweep = 0; sweep < 2; sweep++) {
+      const Tune start = best;
+      for (const Knob &k : knobs) {
+        const Tune base = best;
+        for (int v : k.values) {
+          Tune c = base;
+          k.set(c, v);
+          if (same(c, base)) continue;
+          float ms = 0.f;
+          CL_TRY(time_tune(c, iterations, e0, e1, s, &ms));
+          if (ms < 0.99f * best_ms) {
+            best = c;
+            best_ms = ms;
+          }
         }
       }
+      if (same(best, start)) break;
     }
     apply_tune(best);
     cache[key] = best;
@@ -1567,7 +1584,7 @@ class FastRL : public RLEngine {
   hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};  // see capture()
   Bufs b_{};
   int nparts_ = 0;
-  DevBuf obs_, u_, S_, otf_, parts_, scal_;
+  DevBuf obs_, u_, S_, K_, parts_, scal_;
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_next_ = 0;
