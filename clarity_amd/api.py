@@ -97,6 +97,7 @@ _SIGS = [
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
       ctypes.c_int, ctypes.c_int]),
     ("ClarityEx_RLPlanEngine", ctypes.c_char_p, [ctypes.c_void_p]),
+    ("ClarityEx_RLPlanWorkDims", _R, [ctypes.c_void_p, ctypes.POINTER(Dim3)]),
     ("ClarityEx_Synchronize", _R, []),
     ("ClarityEx_DeviceName", ctypes.c_char_p, []),
     ("ClarityEx_InjectDeviceOOM", _R, [ctypes.c_int64]),
@@ -321,6 +322,15 @@ class RLPlan:
     @property
     def engine(self) -> str:
         return lib().ClarityEx_RLPlanEngine(self.h).decode()
+
+    @property
+    def work_shape(self):
+        """(nz, ny, nx) of the FFT domain the engine transforms over
+        (ClarityEx_RLPlanWorkDims)."""
+        d = Dim3()
+        _check(lib().ClarityEx_RLPlanWorkDims(self.h, ctypes.byref(d)),
+               "ClarityEx_RLPlanWorkDims")
+        return (d.z, d.y, d.x)
 
     def set_inputs(self, img, kernel):
         img = _vol(img)

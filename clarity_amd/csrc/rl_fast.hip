@@ -1143,6 +1143,11 @@ Tune pick_tune(const Geom &g) {
     t.xe[j] = xe;
     t.xt[j] = 16;
   }
+  // 3 * 2^k rows (12 elements per thread) need ~140-160 VGPRs in x_mid and
+  // x_last_fwd: with 8-row tiles three 256-thread workgroups share a CU,
+  // with 16 rows one 512-thread workgroup (measured at M = 384: x_mid 160
+  // against 217 us, x_last_fwd 192 against 228 us)
+  if ((g.nx / 2) % 3 == 0) t.xt[0] = t.xt[1] = t.xt[2] = 8;
   if (g.nx / 2 == 512) {
     t.xt[0] = t.xt[1] = t.xt[2] = 8;
     // Measured at 512x512x128: 16-row tiles read 128-byte spans of every
@@ -1595,7 +1600,68 @@ class FastRL : public RLEngine {
 
 }  // namespace
 
-std::unique_ptr<RLEngine> make_fast_rl(const Work &geom) {
+// Work domain of the fused engine for an RL plan.
+//
+// Along each axis the reference transforms over n = i + k (image plus
+// kernel, src/MaximumLikelihoodDeconvolve.cxx:271-289).  Every quantity an
+// RL iteration keeps is zero outside the image corner [0, i) (the estimate
+// starts as the zero-padded observation and every update multiplies it; the
+// ratio is obs/s1 with obs = 0 there) and s1, s2 are read only inside it.
+// So each circular convolution is only evaluated at offsets d = p - q with
+// p, q in [0, i), |d| < i, and it equals the reference's wherever the
+// kernel, shifted by -k/2 to occupy offsets [-(k/2), k - k/2), cannot reach
+// such a d through the wrap-around: for a transform length N that holds iff
+// N >= i + k/2 (positive offsets up to i - 1 must stay below N - k/2, and
+// negative ones down to -(i - 1) above k - k/2 - 1 - N).  The reference's
+// n = i + k satisfies it; so does any such N, with the same RL iterates in
+// the image corner (up to the float rounding of a different FFT length).
+// The engine also keeps N >= k, so that the kz kernel planes it stores
+// never overlap, and takes the smallest such N it has kernels for: at a same-size
+// kernel 1.5 i instead of 2 i per axis (768x768x192 instead of
+// 1024x1024x256 at 512x512x128, 2.4x fewer work voxels), and shapes whose
+// i + k is no size of the engine's still run on it.  CLARITY_REFERENCE_DOMAIN
+// keeps n = i + k (when the engine has that size).
+namespace {
+
+int smallest_size(long long need, const int *sizes, int count) {
+  for (int j = 0; j < count; j++)
+    if (sizes[j] >= need) return sizes[j];
+  return 0;
+}
+
+bool rl_domain(const Work &ref, Dims *wd) {
+  if (std::getenv("CLARITY_REFERENCE_DOMAIN")) {
+    *wd = ref.wd;
+    return true;
+  }
+  static const int xs[] = {128, 192, 256, 384, 512, 768, 1024};
+  static const int ys[] = {128, 192, 256, 384, 512, 768, 1024, 2048};
+  static const int zs[] = {64, 96, 128, 192, 256, 512};
+  auto need = [](int i, int k) {
+    const long long a = (long long)i + k / 2;
+    return a > k ? a : (long long)k;
+  };
+  wd->x = smallest_size(need(ref.id.x, ref.kd.x), xs, 7);
+  wd->y = smallest_size(need(ref.id.y, ref.kd.y), ys, 8);
+  wd->z = smallest_size(need(ref.id.z, ref.kd.z), zs, 6);
+  // z_mod reads and writes only the iz image and kz kernel planes whatever
+  // nz is, so a longer z transform costs FFT arithmetic, not bytes: take
+  // nz = 2 iz when image and kernel are both that half (the same-size
+  // specialisation of the z kernel)
+  const int h = 2 * ref.id.z;
+  if (ref.kd.z == ref.id.z && h >= wd->z && smallest_size(h, zs, 6) == h) wd->z = h;
+  return wd->x && wd->y && wd->z;
+}
+
+}  // namespace
+
+std::unique_ptr<RLEngine> make_fast_rl(const Work &ref) {
+  Work geom;
+  geom.set_geometry(ref);
+  if (!rl_domain(ref, &geom.wd)) return nullptr;
+  geom.V = geom.wd.voxels();
+  geom.C = (long long)geom.wd.z * geom.wd.y * (geom.wd.x / 2 + 1);
+  geom.scale = inv_volume(geom.wd.x, geom.wd.y, geom.wd.z);
   if (!FastRL::supports(geom)) return nullptr;
   return std::unique_ptr<RLEngine>(new FastRL(geom));
 }

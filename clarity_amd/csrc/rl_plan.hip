@@ -224,11 +224,16 @@ ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
     if (rc == CLARITY_SUCCESS) rc = p->eng->reset(nullptr, s);
     p->have_inputs = true;
   }
-  // interleave: one iteration of every volume at a time, so all streams stay
-  // busy until the end
-  for (int it = 0; it < iterations && rc == CLARITY_SUCCESS; it++)
+  // interleave in rounds of a few iterations of every volume, so all streams
+  // stay busy until the end; within a round a volume's iterations chain
+  // (the fused engine's x_last runs into the next x_fwd) and replay its
+  // captured graphs
+  constexpr int kRound = 10;
+  for (int done = 0; done < iterations && rc == CLARITY_SUCCESS; done += kRound) {
+    const int n = iterations - done < kRound ? iterations - done : kRound;
     for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++)
-      rc = vols[v].plan->eng->iterate(1, streams[v % nstreams]);
+      rc = vols[v].plan->eng->iterate(n, streams[v % nstreams]);
+  }
   for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++) {
     hipStream_t s = streams[v % nstreams];
     const Work &w = vols[v].plan->geom;
@@ -362,6 +367,15 @@ int ClarityEx_RLPlanTimings(ClarityEx_RLPlan plan, int *count, double *ms,
 
 const char *ClarityEx_RLPlanEngine(ClarityEx_RLPlan plan) {
   return plan ? plan->eng->name() : "";
+}
+
+ClarityResult_t ClarityEx_RLPlanWorkDims(ClarityEx_RLPlan plan, Clarity_Dim3 *workDim) {
+  if (!plan || !workDim) return CLARITY_INVALID_ARGUMENT;
+  const Dims d = plan->eng->work_dims();
+  workDim->x = d.x;
+  workDim->y = d.y;
+  workDim->z = d.z;
+  return CLARITY_SUCCESS;
 }
 
 }  // extern "C"

@@ -24,6 +24,8 @@ class RLEngine {
   // Device address of the last rescale factor.
   virtual const float *scale_ptr() const = 0;
   virtual const char *name() const = 0;
+  // The FFT work domain the engine transforms over.
+  Dims work_dims() const { return w_.wd; }
   virtual int describe(const char **names, double *bytes, int *launches,
                        int cap) {
     (void)names; (void)bytes; (void)launches; (void)cap;

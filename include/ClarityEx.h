@@ -125,6 +125,17 @@ C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanAutotune(ClarityEx_RLPlan plan, int i
  * pipeline) or "generic". */
 C_FUNC_DEF const char *ClarityEx_RLPlanEngine(ClarityEx_RLPlan plan);
 
+/* The FFT work domain the plan's engine transforms over.  The generic
+ * engine uses the reference's imageDim + kernelDim per axis
+ * (src/MaximumLikelihoodDeconvolve.cxx:271-289); the fused engine the
+ * smallest of its sizes >= max(kernel, image + kernel/2), over which the RL
+ * iterates in the image are the same (nothing outside the image is kept,
+ * and the kernel cannot wrap into the offsets the image uses), with nz = 2
+ * image z when image and kernel share their depth.  CLARITY_REFERENCE_DOMAIN
+ * makes the fused engine keep imageDim + kernelDim where it can. */
+C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanWorkDims(ClarityEx_RLPlan plan,
+                                                    Clarity_Dim3 *workDim);
+
 /* ---- learned warm start ---------------------------------------------- */
 
 /* A 3-level 3D U-Net (channels 16/32/64, 3x3x3 convolutions, ReLU, skip

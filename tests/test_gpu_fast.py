@@ -63,6 +63,10 @@ SHAPES = [
     ((48, 96, 96), (48, 96, 96)),      # work 96 x 192 x 192: radix 3
     ((32, 192, 64), (32, 192, 64)),    # work 64 x 384 x 128: radix 6 in y
     ((40, 70, 150), (56, 122, 42)),    # work 96 x 192 x 192 from odd dims
+    # kernel deeper than the image: the kernel's planes z < 78 and z >= 178
+    # of 256, general z kernel with 16 elements per thread
+    ((100, 64, 64), (156, 64, 64)),    # work 256 x 128 x 128
+    ((150, 64, 64), (362, 64, 64)),    # work 512 x 128 x 128
 ]
 
 
@@ -81,16 +85,45 @@ def test_fast_vs_oracle(clarity_gpu, ishape, kshape):
 
 
 def test_fast_selected_only_for_supported_shapes(clarity_gpu):
+    """The fused engine transforms over the smallest of its sizes that is
+    at least max(k, i + k/2) per axis, so shapes whose i + k is no size of
+    it run on it too; beyond its largest sizes the generic engine runs.
+    With CLARITY_REFERENCE_DOMAIN it keeps i + k."""
     api = clarity_gpu
-    p = api.RLPlan((32, 64, 64), (32, 64, 64))
-    assert p.engine == "fast"
-    p.close()
-    p = api.RLPlan((32, 64, 64), (31, 64, 64))   # nz = 63: generic
-    assert p.engine == "generic"
-    p.close()
-    p = api.RLPlan((16, 100, 37), (16, 28, 91))  # nz = 32 not in the fast set
-    assert p.engine == "generic"
-    p.close()
+    for ishape, kshape, engine, work in (
+            ((32, 64, 64), (32, 64, 64), "fast", (64, 128, 128)),
+            ((32, 64, 64), (31, 64, 64), "fast", (64, 128, 128)),   # i + k = 63 in z
+            ((16, 100, 37), (16, 28, 91), "fast", (64, 128, 128)),
+            ((128, 512, 512), (128, 512, 512), "fast", (256, 768, 768)),
+            ((16, 64, 1100), (4, 4, 4), "generic", (20, 68, 1104))):  # x: 1102 > 1024
+        p = api.RLPlan(ishape, kshape)
+        assert (p.engine, p.work_shape) == (engine, work), (ishape, kshape)
+        p.close()
+    os.environ["CLARITY_REFERENCE_DOMAIN"] = "1"
+    try:
+        for kz, engine in ((32, "fast"), (31, "generic")):
+            p = api.RLPlan((32, 64, 64), (kz, 64, 64))
+            assert p.engine == engine, kz
+            p.close()
+    finally:
+        os.environ.pop("CLARITY_REFERENCE_DOMAIN", None)
+
+
+@pytest.mark.parametrize("ishape,kshape", [
+    ((32, 64, 64), (31, 64, 64)),    # reference domain 63 x 128 x 128
+    ((16, 100, 37), (16, 28, 91)),   # reference domain 32 x 128 x 128
+])
+def test_fast_on_shapes_outside_its_sizes(clarity_gpu, ishape, kshape):
+    """Shapes whose reference work domain (i + k) the fused engine has no
+    kernels for run on it over a larger alias-free domain, with the
+    reference's results (the oracle transforms over i + k)."""
+    api = clarity_gpu
+    obs, psf = _case(ishape, kshape, 8)
+    got, engine = run_plan(api, obs, psf, 6)
+    assert engine == "fast"
+    err = rel_l2(got, oracle.ml(obs, psf, 6))
+    print(f"[fast] image {ishape} kernel {kshape}, 6 it: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-4
 
 
 def test_fast_matches_generic_100_iterations_cfg1(clarity_gpu):
