@@ -6,9 +6,10 @@
 // Tensors are channels-last bfloat16 ([z][y][x][c], c fastest); weights are
 // float32 masters with bfloat16 copies in two layouts, [tap][co][ci] for the
 // forward pass and [tap][ci][co] for the input gradient.  Every 3x3x3
-// convolution is an implicit GEMM on v_mfma_f32_16x16x16_bf16 (float32
-// accumulation) over K = 27 taps x input channels in chunks of 16; the
-// forward pass stages input tiles in LDS.  The weight gradient is the same
+// convolution is an implicit GEMM (float32 accumulation) over K = 27 taps x
+// input channels in chunks of 16: the forward pass stages input tiles in
+// LDS and runs v_mfma_f32_16x16x32_bf16 on two taps at a time, the input
+// gradient v_mfma_f32_16x16x16_bf16 on one.  The weight gradient is the same
 // GEMM with K = voxels, staged through LDS, reduced over voxel chunks in a
 // fixed order.
 //
@@ -29,6 +30,7 @@ namespace clarity {
 namespace {
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short bf16;
 
@@ -57,13 +59,13 @@ __device__ __forceinline__ void tap_offset(int tap, int &dz, int &dy, int &dx) {
 // Y = relu(conv(X, W) + b).  A block computes a 64 (x) by 4 (y) tile of
 // output voxels of one z plane, one wavefront per row, all COUT channels.
 // Input channels go in chunks of 16: the chunk's (3 x 6 x 66)-voxel input
-// tile is staged in LDS as four planes of channel quads ([quad][voxel], 8 B
-// each, so 16 consecutive voxels of one quad are 128 contiguous bytes), and
-// every tap reads its fragments from there.  The MFMA computes the
-// transposed product: operand A = weights (row = output channel,
-// k = input channel), operand B = input (column = voxel), D row = output
-// channel 4*(lane/16) + j, column = voxel lane%16, so each lane stores four
-// consecutive output channels of one voxel (8 B).
+// tile is staged in LDS as two planes of channel octets ([octet][voxel],
+// 16 B each, one 16-byte read per MFMA operand), and every tap reads its
+// fragments from there.  The MFMA computes the transposed product: operand
+// A = weights (row = output channel, k = (tap, input channel)), operand B =
+// input (column = voxel), D row = output channel 4*(lane/16) + j, column =
+// voxel lane%16, so each lane stores four consecutive output channels of
+// one voxel (8 B).
 //
 // With UP the input is the channel concatenation up(Xc) | S of a coarse
 // tensor Xc (CA channels, nearest-neighbour 2x upsampled on the fly) and a
@@ -72,74 +74,51 @@ constexpr int kXT = 64, kYT = 4;                                // output tile
 constexpr int kTX = kXT + 2, kTY = kYT + 2, kTZ = 3;             // input tile
 constexpr int kTV = kTX * kTY * kTZ;                            // 1188 voxels
 
-template <int CIN, int COUT, int CA, bool UP>
-__global__ void __launch_bounds__(256)
-    k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
-               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
-               bf16 *__restrict__ Y, Vol v) {
-  static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
-                "channel chunks of 16");
-  __shared__ uint2 tile[4][kTV];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m = lane & 15, kq = lane >> 4;
-  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z = blockIdx.z;
-  f32x4 acc[4][COUT / 16];
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-    for (int ct = 0; ct < COUT / 16; ct++) acc[mt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int c0 = 0; c0 < CIN; c0 += 16) {
-    __syncthreads();  // every wavefront is done with the previous chunk
-    // stage: two 16-byte loads (8 channels each) per tile voxel
-    for (int i = threadIdx.x; i < 2 * kTV; i += 256) {
-      const int vox = i >> 1, half = i & 1;
-      const int tx = vox % kTX, t2 = vox / kTX;
-      const int ty = t2 % kTY, tz = t2 / kTY;
-      const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z + tz - 1;
-      uint4 val = {0u, 0u, 0u, 0u};
-      if (gx >= 0 && gx < v.W && gy >= 0 && gy < v.H && gz >= 0 && gz < v.D) {
-        const int c = c0 + 8 * half;
-        const bf16 *p;
-        if (!UP) {
-          p = X + (((long long)gz * v.H + gy) * v.W + gx) * CIN + c;
-        } else if (c < CA) {
-          const int H2 = v.H >> 1, W2 = v.W >> 1;
-          p = X + (((long long)(gz >> 1) * H2 + (gy >> 1)) * W2 + (gx >> 1)) * CA + c;
-        } else {
-          p = S + (((long long)gz * v.H + gy) * v.W + gx) * (CIN - CA) + (c - CA);
-        }
-        val = *reinterpret_cast<const uint4 *>(p);
-      }
-      tile[2 * half][vox] = make_uint2(val.x, val.y);
-      tile[2 * half + 1][vox] = make_uint2(val.z, val.w);
-    }
-    __syncthreads();
-    for (int tap = 0; tap < 27; tap++) {
-      int dz, dy, dx;
-      tap_offset(tap, dz, dy, dx);
-      s16x4 w[COUT / 16];
-#pragma unroll
-      for (int ct = 0; ct < COUT / 16; ct++)
-        w[ct] = *reinterpret_cast<const s16x4 *>(
-            Wf + ((long long)tap * COUT + ct * 16 + m) * CIN + c0 + 4 * kq);
-      const int rb = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
-#pragma unroll
-      for (int mt = 0; mt < 4; mt++) {
-        const uint2 t = tile[kq][rb + mt * 16 + m];
-        s16x4 a;
-        a[0] = (short)(t.x & 0xffffu);
-        a[1] = (short)(t.x >> 16);
-        a[2] = (short)(t.y & 0xffffu);
-        a[3] = (short)(t.y >> 16);
-#pragma unroll
-        for (int ct = 0; ct < COUT / 16; ct++)
-          acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w[ct], a, acc[mt][ct], 0, 0, 0);
-      }
-    }
-  }
-  const int y = y0 + wave;
-  if (y >= v.H) return;
+// The network's output layer fused into the last convolution (inference):
+// out[v] = obs[v]/mean + b + sum_c w[c] d1[v][c], d1 rounded to bfloat16 as
+// the stored activation would be; divided back into the observation's
+// units (after a ReLU) when `rescale`.
+struct HeadArgs {
+  const float *obs = nullptr, *inv_mean = nullptr, *w = nullptr;  // w: 16 + bias
+  float *out = nullptr;
+  int rescale = 0;
+};
+
+// Epilogue of an output tile: y = relu(acc + bias) for the wavefront's row
+// (D row = channel 4*(lane/16) + j, column = voxel lane%16 of each of the
+// four 16-voxel groups), stored as bfloat16 -- or, with HEAD (COUT = 16),
+// reduced over the channels by the head: each lane sums its 4 channels,
+// two lane exchanges sum the 4 lanes of a voxel.
+template <int COUT, bool HEAD>
+__device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[4][COUT / 16],
+                                              const float *__restrict__ bias,
+                                              bf16 *__restrict__ Y, const HeadArgs &h, Vol v,
+                                              int x0, int y, int z, int m, int kq) {
   const long long rowbase = ((long long)z * v.H + y) * v.W;
+  if constexpr (HEAD) {
+    static_assert(COUT == 16, "the head reads 16 channels");
+    const float im = h.inv_mean[0];
+    const float hb = h.w[16];
+    const float4 bb = *reinterpret_cast<const float4 *>(bias + 4 * kq);
+    const float4 ww = *reinterpret_cast<const float4 *>(h.w + 4 * kq);
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const float d0 = bf2f(f2bf(relu(acc[mt][0][0] + bb.x)));
+      const float d1 = bf2f(f2bf(relu(acc[mt][0][1] + bb.y)));
+      const float d2 = bf2f(f2bf(relu(acc[mt][0][2] + bb.z)));
+      const float d3 = bf2f(f2bf(relu(acc[mt][0][3] + bb.w)));
+      float p = ww.x * d0 + ww.y * d1 + ww.z * d2 + ww.w * d3;
+      p += __shfl_xor(p, 16);
+      p += __shfl_xor(p, 32);
+      const int xo = x0 + mt * 16 + m;
+      if (kq == 0 && xo < v.W) {
+        float r = h.obs[rowbase + xo] * im + (hb + p);
+        if (h.rescale) r = (r > 0.f ? r : 0.f) / im;
+        h.out[rowbase + xo] = r;
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int ct = 0; ct < COUT / 16; ct++) {
     const int co = ct * 16 + 4 * kq;
@@ -157,6 +136,178 @@ __global__ void __launch_bounds__(256)
       }
     }
   }
+}
+
+template <int CIN, int COUT, int CA, bool UP, bool HEAD = false>
+__global__ void __launch_bounds__(256)
+    k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
+               bf16 *__restrict__ Y, Vol v, HeadArgs h) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
+                "channel chunks of 16");
+  __shared__ uint4 tile[2][kTV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z = blockIdx.z;
+  f32x4 acc[4][COUT / 16];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++) acc[mt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < CIN; c0 += 16) {
+    // stage: two 16-byte loads (8 channels each) per tile voxel.  All of a
+    // thread's loads are issued before its first LDS store (the counter of
+    // outstanding loads is in order, so a store loop that waits for each
+    // load in turn serialises them); addresses are clamped into the volume
+    // and values outside it cleared by a bit mask, not a branch around the
+    // load.  With CA a multiple of 16 a chunk lies wholly in the upsampled
+    // or the fine part of an UP input.
+    constexpr int NI = (2 * kTV + 255) / 256;
+    const bool coarse = UP && c0 < CA;
+    uint4 stg[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+      const int i0 = threadIdx.x + 256 * j;
+      const int i = i0 < 2 * kTV ? i0 : 2 * kTV - 1;
+      const int vox = i >> 1, half = i & 1;
+      const int tx = vox % kTX, t2 = vox / kTX;
+      const int ty = t2 % kTY, tz = t2 / kTY;
+      const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z + tz - 1;
+      // bitwise, not short-circuit: no branches between the loads
+      const bool in = (gx >= 0) & (gx < v.W) & (gy >= 0) & (gy < v.H) & (gz >= 0) & (gz < v.D);
+      const int cx = min(max(gx, 0), v.W - 1), cy = min(max(gy, 0), v.H - 1);
+      const int cz = min(max(gz, 0), v.D - 1);
+      const int c = c0 + 8 * half;
+      const bf16 *p;
+      if (!UP) {
+        p = X + (((long long)cz * v.H + cy) * v.W + cx) * CIN + c;
+      } else if (coarse) {
+        const int H2 = v.H >> 1, W2 = v.W >> 1;
+        p = X + (((long long)(cz >> 1) * H2 + (cy >> 1)) * W2 + (cx >> 1)) * CA + c;
+      } else {
+        p = S + (((long long)cz * v.H + cy) * v.W + cx) * (CIN - CA) + (c - CA);
+      }
+      const uint4 val = *reinterpret_cast<const uint4 *>(p);
+      const unsigned m = in ? 0xffffffffu : 0u;
+      stg[j] = make_uint4(val.x & m, val.y & m, val.z & m, val.w & m);
+    }
+    __syncthreads();  // every wavefront is done with the previous chunk
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+      const int i = threadIdx.x + 256 * j;
+      if (i < 2 * kTV) tile[i & 1][i >> 1] = stg[j];
+    }
+    __syncthreads();
+    // K = 32 per MFMA: two taps of the chunk's 16 channels.  Lane group
+    // g = lane/16 supplies tap 2p + g/2, channels 8 (g % 2) .. + 7, in both
+    // operands, so whatever order the hardware gives the 32 k, weights and
+    // inputs pair up; the 28th tap has zero weights (its inputs are read at
+    // tap 26's offsets).
+    const int oct = kq & 1;
+    for (int p = 0; p < 14; p++) {
+      const int tp = 2 * p + (kq >> 1);
+      const bool real = tp < 27;
+      int dz, dy, dx;
+      tap_offset(real ? tp : 26, dz, dy, dx);
+      bf16x8 w[COUT / 16];
+#pragma unroll
+      for (int ct = 0; ct < COUT / 16; ct++) {
+        const uint4 wv = *reinterpret_cast<const uint4 *>(
+            Wf + ((long long)(real ? tp : 26) * COUT + ct * 16 + m) * CIN + c0 + 8 * oct);
+        const unsigned z0 = real ? 0xffffffffu : 0u;
+        w[ct] = __builtin_bit_cast(bf16x8, make_uint4(wv.x & z0, wv.y & z0, wv.z & z0, wv.w & z0));
+      }
+      const int rb = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a = __builtin_bit_cast(bf16x8, tile[oct][rb + mt * 16 + m]);
+#pragma unroll
+        for (int ct = 0; ct < COUT / 16; ct++)
+          acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[ct], a, acc[mt][ct], 0, 0, 0);
+      }
+    }
+  }
+  const int y = y0 + wave;
+  if (y >= v.H) return;  // a whole wavefront (one row)
+  conv_epilogue<COUT, HEAD>(acc, bias, Y, h, v, x0, y, z, m, kq);
+}
+
+// The first convolution for inference: the network input (obs/mean,
+// sigma_xy/4, sigma_z/8, then 13 zero channels) is built in the staging
+// from the observation itself, and only its 4 leading channels are kept
+// (8 B per tile voxel).  K = 32 per MFMA = 8 taps x 4 channels: lane group
+// g supplies taps 8p + 2g and 8p + 2g + 1, so 4 MFMAs cover the 27 taps
+// (5 with zero weights) where the 16-channel layout needs 14.  The same
+// products as k_conv_fwd on the materialised input (its other channels are
+// zero).  Weights Wf as k_conv_fwd's with CIN = 16.
+template <int COUT>
+__global__ void __launch_bounds__(256)
+    k_conv_in(const float *__restrict__ obs, const float *__restrict__ inv_mean, float sxy,
+              float sz, const bf16 *__restrict__ Wf, const float *__restrict__ bias,
+              bf16 *__restrict__ Y, Vol v) {
+  __shared__ uint2 tile[kTV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z = blockIdx.z;
+  const float im = inv_mean[0];
+  const unsigned w0hi = (unsigned)f2bf(sxy * 0.25f) << 16;  // channel 1
+  const unsigned w1 = (unsigned)f2bf(sz * 0.125f);          // channel 2 (3: zero)
+  constexpr int NI = (kTV + 255) / 256;
+  float ov[NI];
+  bool inb[NI];
+#pragma unroll
+  for (int j = 0; j < NI; j++) {  // all loads first (clamped, no branches)
+    const int i0 = threadIdx.x + 256 * j;
+    const int i = i0 < kTV ? i0 : kTV - 1;
+    const int tx = i % kTX, t2 = i / kTX;
+    const int ty = t2 % kTY, tz = t2 / kTY;
+    const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z + tz - 1;
+    inb[j] = (gx >= 0) & (gx < v.W) & (gy >= 0) & (gy < v.H) & (gz >= 0) & (gz < v.D);
+    const int cx = min(max(gx, 0), v.W - 1), cy = min(max(gy, 0), v.H - 1);
+    const int cz = min(max(gz, 0), v.D - 1);
+    ov[j] = obs[((long long)cz * v.H + cy) * v.W + cx];
+  }
+#pragma unroll
+  for (int j = 0; j < NI; j++) {
+    const int i = threadIdx.x + 256 * j;
+    const unsigned mk = inb[j] ? 0xffffffffu : 0u;
+    if (i < kTV) tile[i] = make_uint2(((unsigned)f2bf(ov[j] * im) | w0hi) & mk, w1 & mk);
+  }
+  __syncthreads();
+  f32x4 acc[4][COUT / 16];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++) acc[mt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    const int ta = 8 * p + 2 * kq, tb = ta + 1;
+    const unsigned ma = ta < 27 ? 0xffffffffu : 0u, mb = tb < 27 ? 0xffffffffu : 0u;
+    const int ca = ta < 27 ? ta : 26, cb = tb < 27 ? tb : 26;
+    int dz, dy, dx;
+    tap_offset(ca, dz, dy, dx);
+    const int ra = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
+    tap_offset(cb, dz, dy, dx);
+    const int rb = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
+    bf16x8 w[COUT / 16];
+#pragma unroll
+    for (int ct = 0; ct < COUT / 16; ct++) {
+      const uint2 wa = *reinterpret_cast<const uint2 *>(Wf + ((long long)ca * COUT + ct * 16 + m) * 16);
+      const uint2 wb = *reinterpret_cast<const uint2 *>(Wf + ((long long)cb * COUT + ct * 16 + m) * 16);
+      w[ct] = __builtin_bit_cast(bf16x8, make_uint4(wa.x & ma, wa.y & ma, wb.x & mb, wb.y & mb));
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const uint2 a0 = tile[ra + mt * 16 + m], a1 = tile[rb + mt * 16 + m];
+      const bf16x8 a = __builtin_bit_cast(bf16x8, make_uint4(a0.x, a0.y, a1.x, a1.y));
+#pragma unroll
+      for (int ct = 0; ct < COUT / 16; ct++)
+        acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[ct], a, acc[mt][ct], 0, 0, 0);
+    }
+  }
+  const int y = y0 + wave;
+  if (y >= v.H) return;
+  conv_epilogue<COUT, false>(acc, bias, Y, HeadArgs{}, v, x0, y, z, m, kq);
 }
 
 // GX = conv^T(G * [Y > 0], W): input gradient of a ReLU convolution.
@@ -538,7 +689,14 @@ void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf
                 hipStream_t s) {
   hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
                      dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
-                     X, S, Wf, b, Y, v);
+                     X, S, Wf, b, Y, v, HeadArgs{});
+}
+// the last convolution (16 -> 16) with the head fused into its epilogue
+void conv_fwd_head(const bf16 *X, const bf16 *Wf, const float *b, const HeadArgs &h, Vol v,
+                   hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_fwd<16, 16, 0, false, true>),
+                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
+                     X, nullptr, Wf, b, nullptr, v, h);
 }
 template <int CIN, int COUT>
 void conv_bwd_t(const bf16 *G, const bf16 *Yo, const bf16 *X, const bf16 *Wb, bf16 *GX,
@@ -694,9 +852,12 @@ class UNet {
   }
 
   // forward on a device observation; leaves activations for backward (and,
-  // with keep_concat, the decoder's concatenated inputs)
+  // with keep_concat, the decoder's concatenated inputs).  With head_out
+  // (inference) the network input is built inside the first convolution
+  // and the head runs in the last one's epilogue, writing head_out; neither
+  // the 16-channel input nor the last activation is stored.
   ClarityResult_t forward(const float *obs_d, float sxy, float sz, bool keep_concat,
-                          hipStream_t s) {
+                          hipStream_t s, float *head_out = nullptr, bool rescale = false) {
     const Vol v = v0_;
     const Vol v1{v.D / 2, v.H / 2, v.W / 2}, v2{v.D / 4, v.H / 4, v.W / 4};
     const Vol lv[3] = {v, v1, v2};
@@ -704,8 +865,9 @@ class UNet {
     CL_TRY(k_sum_partials(obs_d, v.n(), parts_.d(), kParts, s));
     CL_TRY(k_finish_sum(parts_.d(), kParts, invm_.f(), nullptr, nullptr, s));
     hipLaunchKernelGGL(k_recip_mean, dim3(1), dim3(1), 0, s, invm_.f(), (float)v.n());
-    hipLaunchKernelGGL(k_input, dim3(grid_for(v.n())), dim3(256), 0, s, obs_d, invm_.f() + 1,
-                       sxy, sz, (bf16 *)in_.p, v.n());
+    if (!head_out)
+      hipLaunchKernelGGL(k_input, dim3(grid_for(v.n())), dim3(256), 0, s, obs_d,
+                         invm_.f() + 1, sxy, sz, (bf16 *)in_.p, v.n());
     const bf16 *src[10];
     src[0] = (bf16 *)in_.p;
     src[1] = (bf16 *)act_[0].p;
@@ -728,6 +890,22 @@ class UNet {
       const float *b = p_.f() + off_[10 + i];
       const bf16 *Wf = (bf16 *)wf_[i].p;
       const Vol vv = lv[kLevel[i]];
+      if (i == 0 && head_out) {
+        hipLaunchKernelGGL((k_conv_in<16>),
+                           dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256),
+                           0, s, obs_d, invm_.f() + 1, sxy, sz, Wf, b, Y, v);
+        continue;
+      }
+      if (i == 9 && head_out) {
+        HeadArgs h;
+        h.obs = obs_d;
+        h.inv_mean = invm_.f() + 1;
+        h.w = p_.f() + off_[20];
+        h.out = head_out;
+        h.rescale = rescale ? 1 : 0;
+        conv_fwd_head((bf16 *)act_[8].p, Wf, b, h, v, s);
+        continue;
+      }
       // decoder inputs up(coarse) | skip: read in place for inference; the
       // training pass materialises them (the weight gradient reads them)
       if (i == 6 && !keep_concat) {
@@ -869,8 +1047,8 @@ ClarityResult_t unet_run(ClarityEx_UNet h, const float *obs, const float *tgt,
       tgt_d = b.f();
     }
   }
-  CL_TRY(n.forward(obs_d, sxy, sz, train, s));
   if (train) {
+    CL_TRY(n.forward(obs_d, sxy, sz, true, s));
     CL_TRY(n.head(obs_d, n.out_.f(), false, s));
     CL_TRY(n.backward(tgt_d, lr, loss, s));
     if (grads)
@@ -878,10 +1056,10 @@ ClarityResult_t unet_run(ClarityEx_UNet h, const float *obs, const float *tgt,
     return CLARITY_SUCCESS;
   }
   if (on_device) {
-    CL_TRY(n.head(obs_d, out, true, s));
+    CL_TRY(n.forward(obs_d, sxy, sz, false, s, out, true));
   } else {
     CL_TRY(o.alloc(sizeof(float) * nv));
-    CL_TRY(n.head(obs_d, o.f(), true, s));
+    CL_TRY(n.forward(obs_d, sxy, sz, false, s, o.f(), true));
     CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * nv, hipMemcpyDeviceToHost, s));
   }
   CL_HIP(hipStreamSynchronize(s));
