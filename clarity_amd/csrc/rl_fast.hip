@@ -270,9 +270,20 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
 #pragma unroll
     for (int r = 0; r < R; r++) u[r] = v[b + r * NB];
     if constexpr (P > 1) {
+      if constexpr (R == 3 || R == 4) {
+        // one table read and powers by complex products (a few ulp more
+        // error per twiddle, two LDS reads fewer per butterfly: the
+        // 3 * 2^k row passes are bound by LDS instructions)
+        const float2 w1 = tw_load<SIGN>(tw, k * (N / (P * R)));
+        const float2 w2 = cmul(w1, w1);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        if constexpr (R == 4) u[3] = cmul(u[3], cmul(w2, w1));
+      } else {
 #pragma unroll
-      for (int r = 1; r < R; r++)
-        u[r] = cmul(u[r], tw_load<SIGN>(tw, r * k * (N / (P * R))));
+        for (int r = 1; r < R; r++)
+          u[r] = cmul(u[r], tw_load<SIGN>(tw, r * k * (N / (P * R))));
+      }
     }
     dft<R, SIGN>(u);
     if constexpr (LAST) {
@@ -458,33 +469,34 @@ __device__ __forceinline__ void pack_inverse(float2 (&v)[E], const L_t &L, int t
 // by a select, which the compiler would turn back into a branch around the
 // load: without branches between them every load is in flight together.
 // Element e of a thread holds the real positions n0 = 2 (t + T e), n0 + 1.
-// With H (ix == M, a same-size kernel along x) exactly the elements e < E/2
-// lie in the image, known at compile time.
-template <bool H, int E>
+// With HE > 0 (ix == 2 T HE: a row that ends on an element boundary, e.g.
+// HE = E/2 for a same-size kernel with ix = M, or HE = 2E/3 for ix = 2M/3)
+// exactly the elements e < HE lie in the image, known at compile time.
+template <int HE>
 __device__ __forceinline__ bool in_image(int e, int n0, int ix) {
-  if constexpr (H)
-    return e < E / 2;
+  if constexpr (HE > 0)
+    return e < HE;
   else
     return n0 < ix;
 }
-template <bool H, int E>
+template <int HE>
 __device__ __forceinline__ bool second_in_image(int e, int n0, int ix) {
-  if constexpr (H)
-    return e < E / 2;
+  if constexpr (HE > 0)
+    return e < HE;
   else
     return n0 + 1 < ix;
 }
 
-template <int E, int T, bool H = false>
+template <int E, int T, int HE = 0>
 __device__ __forceinline__ void load_row_pairs(float2 (&v)[E], const float *row, bool valid,
                                                int t, int ix) {
-  if constexpr (H) {
+  if constexpr (HE > 0) {
     // rows of a partial tile are transformed alone and never stored, so
     // they need no masking; the elements past the image are zero
 #pragma unroll
     for (int e = 0; e < E; e++)
-      v[e] = (e < E / 2) ? *reinterpret_cast<const float2 *>(row + 2 * (t + T * e))
-                         : make_float2(0.f, 0.f);
+      v[e] = (e < HE) ? *reinterpret_cast<const float2 *>(row + 2 * (t + T * e))
+                      : make_float2(0.f, 0.f);
     (void)valid;
     (void)ix;
   } else if ((ix & 1) == 0) {
@@ -551,7 +563,7 @@ struct XSmem {
   float2 twN[M + 1];
 };
 
-template <int M, int E, int TP, bool H>
+template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
@@ -571,7 +583,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 v[E];
-  load_row_pairs<E, T, H>(v, load_row(u, g, z, y), y < g.iy, t, g.ix);
+  load_row_pairs<E, T, HE>(v, load_row(u, g, z, y), y < g.iy, t, g.ix);
   tm.store(sm.twM);
   tn.store(sm.twN);
   __syncthreads();
@@ -583,7 +595,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-template <int M, int E, int TP, bool H>
+template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
             const float *__restrict__ cptr, const float2 *__restrict__ gtwM,
@@ -610,7 +622,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 o[E];
-  load_row_pairs<E, T, H>(o, load_row(obs, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, HE>(o, load_row(obs, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -629,11 +641,11 @@ __global__ void __launch_bounds__(TP *(M / E))
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     float r0 = 0.f, r1 = 0.f;
-    if (valid && in_image<H, E>(e, n0, g.ix)) {
+    if (valid && in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].x;
       r0 = (fabsf(s) < kDivGuard) ? 0.f : o[e].x / s;
     }
-    if (valid && second_in_image<H, E>(e, n0, g.ix)) {
+    if (valid && second_in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].y;
       r1 = (fabsf(s) < kDivGuard) ? 0.f : o[e].y / s;
     }
@@ -646,7 +658,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
 }
 
-template <int M, int E, int TP, bool H>
+template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
@@ -672,7 +684,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T, H>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, HE>(uo, load_row(u, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -689,8 +701,8 @@ __global__ void __launch_bounds__(TP *(M / E))
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
-    if (valid && in_image<H, E>(e, n0, g.ix)) {
-      const bool second = second_in_image<H, E>(e, n0, g.ix);
+    if (valid && in_image<HE>(e, n0, g.ix)) {
+      const bool second = second_in_image<HE>(e, n0, g.ix);
       const float u0 = (c * uo[e].x) * v[e].x;
       const float u1 = second ? (c * uo[e].y) * v[e].y : 0.f;
       store_row_pair(urow, n0, g.ix, u0, u1);
@@ -714,7 +726,7 @@ __global__ void __launch_bounds__(TP *(M / E))
 // along x (the next iteration's first pass, which needs no scale: c enters
 // at x_mid) goes to S.  The tile's spectrum rows are read completely before
 // any of them is rewritten, and no other workgroup touches them.
-template <int M, int E, int TP, bool H>
+template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
                  const float *__restrict__ cptr, double *__restrict__ partials,
@@ -740,7 +752,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   tm.load(gtwM);
   tn.load(gtwN);
   float2 uo[E];
-  load_row_pairs<E, T, H>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  load_row_pairs<E, T, HE>(uo, load_row(u, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -757,8 +769,8 @@ __global__ void __launch_bounds__(TP *(M / E))
   for (int e = 0; e < E; e++) {
     const int n0 = 2 * (t + T * e);
     float u0 = 0.f, u1 = 0.f;
-    if (valid && in_image<H, E>(e, n0, g.ix)) {
-      const bool second = second_in_image<H, E>(e, n0, g.ix);
+    if (valid && in_image<HE>(e, n0, g.ix)) {
+      const bool second = second_in_image<HE>(e, n0, g.ix);
       u0 = (c * uo[e].x) * v[e].x;
       if (second) u1 = (c * uo[e].y) * v[e].y;
       store_row_pair(urow, n0, g.ix, u0, u1);
@@ -973,36 +985,47 @@ struct Bufs {
 // Number of x tiles (= x_last partial sums) for a tile height.
 int x_tiles(const Geom &g, int tp) { return g.iz * ((g.iy + tp - 1) / tp); }
 
-template <int M, int E, int TP, bool H>
+template <int M, int E, int TP, int HE>
 void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr int NT = TP * (M / E);
   const int nt = x_tiles(g, TP);
   const unsigned grid = (unsigned)(TP == 16 ? nt : (nt + 15) / 16 * 16);
   if (kind == KX_FWD)
-    hipLaunchKernelGGL((k_x_fwd<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
+    hipLaunchKernelGGL((k_x_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
   else if (kind == KX_MID)
-    hipLaunchKernelGGL((k_x_mid<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
+    hipLaunchKernelGGL((k_x_mid<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
   else if (kind == KX_LAST_FWD)
-    hipLaunchKernelGGL((k_x_last_fwd<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
   else
-    hipLaunchKernelGGL((k_x_last<M, E, TP, H>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+    hipLaunchKernelGGL((k_x_last<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
 }
 
-// H: the image fills exactly half the x transform (ix == M, a same-size
-// kernel): the row bounds are known at compile time.
+// With h, a row that ends on an element boundary of every thread (ix =
+// 2 T HE) gets the kernel whose row bounds are compile-time: HE = E/2 (the
+// image is half the x transform, ix == M: a same-size kernel on the
+// reference domain) or HE = 2E/3 (ix == 2M/3: a same-size kernel on the
+// alias-free domain of 1.5 times the image).
+template <int M, int E, int TP>
+void launch_x_h(int he, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
+  if (he == E / 2) return launch_x_t<M, E, TP, E / 2>(kind, g, b, s);
+  if constexpr (E % 3 == 0) {
+    if (he == 2 * E / 3) return launch_x_t<M, E, TP, 2 * E / 3>(kind, g, b, s);
+  }
+  launch_x_t<M, E, TP, 0>(kind, g, b, s);
+}
+
 template <int M, int E>
 void launch_x_e(int tp, bool h, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  if (tp == 8) {
-    if (h) launch_x_t<M, E, 8, true>(kind, g, b, s);
-    else launch_x_t<M, E, 8, false>(kind, g, b, s);
-  } else {
-    if (h) launch_x_t<M, E, 16, true>(kind, g, b, s);
-    else launch_x_t<M, E, 16, false>(kind, g, b, s);
-  }
+  constexpr int T = M / E;
+  const int he = (h && g.ix % (2 * T) == 0) ? g.ix / (2 * T) : 0;
+  if (tp == 8)
+    launch_x_h<M, E, 8>(he, kind, g, b, s);
+  else
+    launch_x_h<M, E, 16>(he, kind, g, b, s);
 }
 
 template <int M>
@@ -1024,7 +1047,7 @@ void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStre
   const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1 : (kind == KX_LAST) ? 2 : 3;
   const int jt = j < 3 ? j : 2;  // x_last_fwd takes x_last's launch shape
   const int e = tu.xe[jt], tp = tu.xt[jt];
-  const bool h = ((tu.xh >> j) & 1) && 2 * g.ix == g.nx;
+  const bool h = (tu.xh >> j) & 1;  // the launcher checks that the rows allow it
   switch (g.nx / 2) {
     case 64: launch_x<64>(e, tp, h, kind, g, b, s); break;
     case 96: launch_x<96>(e, tp, h, kind, g, b, s); break;
@@ -1143,11 +1166,20 @@ Tune pick_tune(const Geom &g) {
     t.xe[j] = xe;
     t.xt[j] = 16;
   }
-  // 3 * 2^k rows (12 elements per thread) need ~140-160 VGPRs in x_mid and
-  // x_last_fwd: with 8-row tiles three 256-thread workgroups share a CU,
-  // with 16 rows one 512-thread workgroup (measured at M = 384: x_mid 160
-  // against 217 us, x_last_fwd 192 against 228 us)
-  if ((g.nx / 2) % 3 == 0) t.xt[0] = t.xt[1] = t.xt[2] = 8;
+  // 3 * 2^k rows (12 elements per thread): the general x_mid and x_last_fwd
+  // need ~132 VGPRs, three 256-thread workgroups per CU with 8-row tiles,
+  // one 512-thread workgroup with 16; rows that end on an element boundary
+  // of every thread (ix = 2 T HE, HE = 6 or 8) need ~121, and with 16-row
+  // tiles two 512-thread workgroups share a CU (measured at M = 384,
+  // ix = 512: x_mid 127 us and x_last_fwd 163 us, against 160 / 190 us for
+  // the general kernels with 8-row tiles)
+  if ((g.nx / 2) % 3 == 0) {
+    const int T = (g.nx / 2) / 12;
+    const int he = g.ix % (2 * T) == 0 ? g.ix / (2 * T) : 0;
+    t.xt[0] = 8;
+    t.xt[1] = t.xt[2] = (he == 6 || he == 8) ? 16 : 8;
+    t.xh = 15;
+  }
   if (g.nx / 2 == 512) {
     t.xt[0] = t.xt[1] = t.xt[2] = 8;
     // Measured at 512x512x128: 16-row tiles read 128-byte spans of every
