@@ -204,6 +204,10 @@ __global__ void __launch_bounds__(256)
     // inputs pair up; the 28th tap has zero weights (its inputs are read at
     // tap 26's offsets).
     const int oct = kq & 1;
+    // unrolled by two, not fully: a full unroll of the 14 pairs holds the
+    // next chunk's loads and weights in ~50 more VGPRs (one wavefront per
+    // SIMD less) for the layers with several input chunks
+#pragma unroll 2
     for (int p = 0; p < 14; p++) {
       const int tp = 2 * p + (kq >> 1);
       const bool real = tp < 27;
