@@ -112,6 +112,9 @@ def test_fast_selected_only_for_supported_shapes(clarity_gpu):
 @pytest.mark.parametrize("ishape,kshape", [
     ((32, 64, 64), (31, 64, 64)),    # reference domain 63 x 128 x 128
     ((16, 100, 37), (16, 28, 91)),   # reference domain 32 x 128 x 128
+    ((1, 128, 100), (1, 9, 7)),      # one plane: 2 x 137 x 107 -> 64 x 256 x 128
+    ((5, 40, 30), (3, 120, 90)),     # kernel wider than the image in y and x
+    ((7, 33, 45), (20, 17, 9)),      # kernel deeper than the image, odd sizes
 ])
 def test_fast_on_shapes_outside_its_sizes(clarity_gpu, ishape, kshape):
     """Shapes whose reference work domain (i + k) the fused engine has no
