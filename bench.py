@@ -22,7 +22,7 @@ Also reported:
                 serial energy sum), timed on this host on a bounded sample.
   learned_warm_start
                 the metric's second half: the committed network + 20 RL
-                iterations against plain RL on the same volume, judged by
+                iterations against plain RL on the same volume, measured against
                 plain RL after 200 iterations (relative L2, I-divergence),
                 with plain RL's time to reach the same error.
 
@@ -95,7 +95,9 @@ class Hip:
 
 
 def make_inputs(api, shape, seed=3):
-    """Config-3 statistics on `shape`; observation convolved on the GPU."""
+    """The filament workload's statistics (2000 beads and 300 filaments per
+    256x256x64, scaled by volume) on `shape`; observation convolved on the
+    GPU."""
     from clarity_amd import synth
     rng = np.random.default_rng(seed)
     base = (64, 256, 256)
@@ -135,8 +137,8 @@ def dense_bytes(shape, passes_real, passes_complex):
 
 def suite(api, threads):
     """Every other driver on the path, GPU vs the CPU oracle on the same
-    inputs (config-2 volume 256x256x64 with a same-size kernel; batched RL
-    on config-4's 48 uneven volumes).  GPU times are whole C-API calls with
+    inputs (a 256x256x64 volume with a same-size kernel; batched RL on 48
+    uneven volumes of 64..256 x 64..256 x 32..96).  GPU times are whole C-API calls with
     host buffers (upload, pad, compute, clip, download); the dense-model
     bandwidth counts only the work-domain sweeps of the algorithm."""
     import oracle
@@ -193,7 +195,7 @@ def suite(api, threads):
     cb = cpu_time(lambda: oracle.blind_rl(obs, k0, 1, 1))
     emit("blind rl 4 outer x 5 inner 256x256x64", g, 20 * cb, 0,
          "cpu: 20 x the cost of a 1 x 1 call")
-    # config 4: 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
+    # 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
     rng = np.random.default_rng(4)
     imgs, ks = [], []
     for _ in range(48):
@@ -355,7 +357,7 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s")
 
     # time-to-accuracy of the learned warm start against plain RL, on the
-    # same volume (network + 20 RL iterations vs plain RL, judged against
+    # same volume (network + 20 RL iterations vs plain RL, measured against
     # plain RL after 200 iterations)
     learned = None
     if not args.no_learned and shape == (128, 512, 512):
@@ -377,7 +379,7 @@ def main():
                 is None else round(ev["plain_iterations_to_same_error"], 1),
                 "time_to_accuracy_speedup": None if ev["time_to_accuracy_speedup"] is None
                 else round(ev["time_to_accuracy_speedup"], 2),
-                "wall_ratio_vs_rl200": round(ev["wall_ratio_vs_judge"], 2),
+                "wall_ratio_vs_rl200": round(ev["wall_ratio_vs_yardstick"], 2),
                 "accepted_at_1e-3": ev["accepted"],
             }
             log(f"[bench] learned {learned}")

@@ -4,14 +4,14 @@ A 3-level 3D U-Net (ClarityEx_UNet*: 16/32/64 channels, 3x3x3 convolutions,
 ReLU, skip connections, bfloat16 weights on MFMA with float32 accumulation)
 maps an observation and its kernel widths to an estimate of a late plain-RL
 iterate; a short RL run (ClarityEx_RLPlanReset + ClarityEx_RLPlanIterate)
-refines it.  Plain RL from the observation is both the judge and the
+refines it.  Plain RL from the observation is both the yardstick and the
 fallback:
 
-* judge -- ``evaluate`` runs plain RL for ``JUDGE_ITERATIONS`` on the same
+* yardstick -- ``evaluate`` runs plain RL for ``YARDSTICK_ITERATIONS`` on the same
   inputs and measures the refined result against it (relative L2 and
   I-divergence), beside the plain-RL error curve and both wall times;
 * fallback -- ``deconvolve`` takes the learned path only when the model
-  card stored with the weights says that the judge accepted it at the
+  card stored with the weights says that the yardstick accepted it at the
   requested tolerance and the kernel widths lie in the trained range;
   otherwise it runs plain RL.
 
@@ -46,8 +46,8 @@ SIGMA_XY = (1.2, 2.5)
 SIGMA_Z = (2.5, 5.0)
 TARGET_ITERATIONS = 180           # network target
 REFINE_ITERATIONS = 20            # RL iterations after the network
-JUDGE_ITERATIONS = 200            # plain RL that the result is held against
-ACCEPT_REL_L2 = 1e-3              # acceptance bar against the judge
+YARDSTICK_ITERATIONS = 200        # plain RL that the result is held against
+ACCEPT_REL_L2 = 1e-3              # acceptance bar against the yardstick
 ACCEPT_IDIV = 1e-3                # relative I-divergence slack
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
@@ -113,7 +113,7 @@ def save(path: str, params, card: dict) -> None:
 
 def accepts(card: dict, sigma_xy: float, sigma_z: float,
             tolerance: float = ACCEPT_REL_L2, idiv_tolerance: float = ACCEPT_IDIV) -> bool:
-    """Whether the learned path may replace plain RL: the judge's measured
+    """Whether the learned path may replace plain RL: the yardstick's measured
     error of the refined result (stored in the card) is within `tolerance`,
     its I-divergence exceeds plain RL's by at most `idiv_tolerance`
     (relative), and the widths are in the trained range."""
@@ -130,14 +130,14 @@ def accepts(card: dict, sigma_xy: float, sigma_z: float,
 
 # --------------------------------------------------------------- training
 
-def make_dataset(first: int, count: int, judge: bool = False, workers: int = 8):
+def make_dataset(first: int, count: int, yardstick: bool = False, workers: int = 8):
     """Observations and plain-RL targets of patches first..first+count-1 on
-    the GPU (torch float32 tensors [count, z, y, x]).  With ``judge`` the
-    plain-RL result after JUDGE_ITERATIONS is returned too."""
+    the GPU (torch float32 tensors [count, z, y, x]).  With ``yardstick`` the
+    plain-RL result after YARDSTICK_ITERATIONS is returned too."""
     import torch
     obs_t = torch.empty((count,) + PATCH, device="cuda")
     tgt_t = torch.empty((count,) + PATCH, device="cuda")
-    ref_t = torch.empty((count,) + PATCH, device="cuda") if judge else None
+    ref_t = torch.empty((count,) + PATCH, device="cuda") if yardstick else None
     sig = np.zeros((count, 2), np.float32)
     plan = api.RLPlan(PATCH, PATCH)
     t0 = time.time()
@@ -152,8 +152,8 @@ def make_dataset(first: int, count: int, judge: bool = False, workers: int = 8):
             plan.reset()
             plan.iterate(TARGET_ITERATIONS)
             plan.estimate_device(tgt_t[i].data_ptr())
-            if judge:
-                plan.iterate(JUDGE_ITERATIONS - TARGET_ITERATIONS)
+            if yardstick:
+                plan.iterate(YARDSTICK_ITERATIONS - TARGET_ITERATIONS)
                 plan.estimate_device(ref_t[i].data_ptr())
             obs_t[i].copy_(torch.from_numpy(p.obs))
             sig[i] = (p.sigma_xy, p.sigma_z)
@@ -243,8 +243,8 @@ def _timed(fn):
 
 def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
              curve_step: int = 10, repeats: int = 3) -> dict:
-    """Judges network + REFINE_ITERATIONS of RL against plain RL after
-    JUDGE_ITERATIONS on one volume (default: the 512x512x128 benchmark
+    """Measures network + REFINE_ITERATIONS of RL against plain RL after
+    YARDSTICK_ITERATIONS on one volume (default: the 512x512x128 benchmark
     volume), with inputs resident on the GPU.  Reports both errors and
     I-divergences, the plain-RL error curve, wall times (median of
     `repeats` after a warm-up) and the time-to-accuracy ratio: plain RL's
@@ -252,7 +252,7 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
     time."""
     import torch
     if case is None:
-        case = synth.make_config(3, convolve=api.Convolve)
+        case = synth.make_workload("filaments-512x512x128", convolve=api.Convolve)
     shape = case.obs.shape
     obs_d = torch.from_numpy(case.obs).cuda()
     psf_d = torch.from_numpy(np.ascontiguousarray(case.psf, np.float32)).cuda()
@@ -276,11 +276,11 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
         fn()
         return float(np.median([_timed(fn) for _ in range(repeats)]))
 
-    t_plain = med(lambda: plain(JUDGE_ITERATIONS))
+    t_plain = med(lambda: plain(YARDSTICK_ITERATIONS))
     t_learned = med(learned)
     t_net = med(lambda: net.predict_device(obs_d.data_ptr(), shape, sigma_xy, sigma_z,
                                            warm.data_ptr()))
-    plain(JUDGE_ITERATIONS)
+    plain(YARDSTICK_ITERATIONS)
     plan.estimate_device(ref.data_ptr())
     nref = float(torch.linalg.vector_norm(ref))
 
@@ -290,7 +290,7 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
     curve = []
     plan.reset()
     done = 0
-    for k in range(curve_step, JUDGE_ITERATIONS + 1, curve_step):
+    for k in range(curve_step, YARDSTICK_ITERATIONS + 1, curve_step):
         plan.iterate(k - done)
         done = k
         plan.estimate_device(cur.data_ptr())
@@ -314,20 +314,20 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
             k_eq = k0 + (k - k0) * (e0 - e_learned) / max(e0 - e, 1e-30)
             break
         prev = (k, e)
-    per_it = t_plain / JUDGE_ITERATIONS
+    per_it = t_plain / YARDSTICK_ITERATIONS
     t_eq = None if k_eq is None else k_eq * per_it
     ev = {
         "volume": list(shape),
         "sigma_xy": sigma_xy, "sigma_z": sigma_z,
         "refine_iterations": REFINE_ITERATIONS,
-        "judge_iterations": JUDGE_ITERATIONS,
+        "yardstick_iterations": YARDSTICK_ITERATIONS,
         "learned_rel_l2": e_learned,
         "network_rel_l2": e_warm,
         "plain_curve": curve,
         "idiv_plain": idiv_p, "idiv_learned": idiv_l,
         "learned_idiv_rel_gap": (idiv_l - idiv_p) / idiv_p,
         "seconds_plain": t_plain, "seconds_learned": t_learned, "seconds_network": t_net,
-        "wall_ratio_vs_judge": t_plain / t_learned,
+        "wall_ratio_vs_yardstick": t_plain / t_learned,
         "plain_iterations_to_same_error": k_eq,
         "time_to_accuracy_speedup": None if t_eq is None else t_eq / t_learned,
         "accepted": bool(e_learned <= ACCEPT_REL_L2 and
@@ -339,19 +339,19 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
 
 # ------------------------------------------------------------ deconvolve
 
-def deconvolve(obs, psf, sigma_xy: float, sigma_z: float, iterations: int = JUDGE_ITERATIONS,
+def deconvolve(obs, psf, sigma_xy: float, sigma_z: float, iterations: int = YARDSTICK_ITERATIONS,
                tolerance: float = ACCEPT_REL_L2, idiv_tolerance: float = ACCEPT_IDIV,
                weights: str = WEIGHTS):
     """Deconvolves `obs` (z, y, x float32, host) with `psf`.  Takes the
     learned path (network + REFINE_ITERATIONS of RL) when the model card
     accepts it at `tolerance` / `idiv_tolerance` for these widths and `iterations` is the
-    judge's count; otherwise plain RL for `iterations` (the fallback).
+    yardstick's count; otherwise plain RL for `iterations` (the fallback).
     Returns (estimate, path) with path "learned" or "plain"."""
     obs = np.ascontiguousarray(obs, np.float32)
     card = None
     if os.path.exists(weights):
         params, card = load(weights)
-    use = (card is not None and iterations == card.get("judge_iterations") and
+    use = (card is not None and iterations == card.get("yardstick_iterations") and
            accepts(card, sigma_xy, sigma_z, tolerance, idiv_tolerance) and
            all(n % 4 == 0 for n in obs.shape))
     plan = api.RLPlan(obs.shape, psf.shape)
@@ -400,7 +400,7 @@ def main(argv=None):
     log("evaluation " + json.dumps({k: v for k, v in ev.items() if k != "plain_curve"}))
     card.update({
         "patch": list(PATCH), "n_train": a.train, "target_iterations": TARGET_ITERATIONS,
-        "refine_iterations": REFINE_ITERATIONS, "judge_iterations": JUDGE_ITERATIONS,
+        "refine_iterations": REFINE_ITERATIONS, "yardstick_iterations": YARDSTICK_ITERATIONS,
         "sigma_xy": list(SIGMA_XY), "sigma_z": list(SIGMA_Z), "training": hist,
         "evaluation": ev,
     })

@@ -91,29 +91,32 @@ class Case:
     obs: np.ndarray
 
 
-def make_config(cfg: int, seed: int | None = None, convolve=np_convolve,
-                shape=None) -> Case:
-    """Synthetic inputs of the benchmark configurations.
+WORKLOADS = ("beads-64x64x32", "filaments-256x256x64", "filaments-512x512x128")
 
-    cfg 1: 64x64x32, 200 beads I~U[100,1000] on background 10, Gaussian PSF
-           sigma_xy 1.5, sigma_z 3.0, same size as the image.
-    cfg 2: 256x256x64, 2000 beads + 300 random-walk filaments (length
-           40..200, +200 per voxel), PSF sigma_xy 2.0, sigma_z 4.5 with 10 %
-           side lobes at z = +-8.
-    cfg 3: 512x512x128 with cfg-2 statistics scaled by volume (x8).
+
+def make_workload(name: str, seed: int | None = None, convolve=np_convolve,
+                  shape=None) -> Case:
+    """Synthetic inputs of the benchmark workloads (default seeds 1, 2, 3).
+
+    beads-64x64x32: 200 beads I~U[100,1000] on background 10, Gaussian PSF
+        sigma_xy 1.5, sigma_z 3.0, same size as the image.
+    filaments-256x256x64: 2000 beads + 300 random-walk filaments (length
+        40..200, +200 per voxel), PSF sigma_xy 2.0, sigma_z 4.5 with 10 %
+        side lobes at z = +-8.
+    filaments-512x512x128: the same statistics scaled by volume (x8).
     ``shape`` overrides the volume (z, y, x) with statistics scaled by volume.
     """
-    rng = np.random.default_rng(cfg if seed is None else seed)
-    if cfg == 1:
+    if name not in WORKLOADS:
+        raise ValueError(f"unknown workload {name}")
+    rng = np.random.default_rng(WORKLOADS.index(name) + 1 if seed is None else seed)
+    if name == "beads-64x64x32":
         base, beads, fils = (32, 64, 64), 200, 0
         sxy, sz, lobe = 1.5, 3.0, 0.0
-    elif cfg in (2, 3):
+    else:
         base, beads, fils = (64, 256, 256), 2000, 300
         sxy, sz, lobe = 2.0, 4.5, 0.1
-        if cfg == 3:
+        if name == "filaments-512x512x128":
             shape = shape or (128, 512, 512)
-    else:
-        raise ValueError(f"unknown config {cfg}")
     shape = tuple(shape) if shape else base
     f = np.prod(shape) / np.prod(base)
     obj = beads_and_filaments(shape, int(round(beads * f)), int(round(fils * f)), rng)
@@ -122,8 +125,8 @@ def make_config(cfg: int, seed: int | None = None, convolve=np_convolve,
     return Case(obj, psf, obs)
 
 
-def blind_initial_kernel(cfg: int, shape) -> np.ndarray:
-    """Blind-ML starting kernel: the config's Gaussian 20 % too wide."""
-    if cfg == 1:
+def blind_initial_kernel(name: str, shape) -> np.ndarray:
+    """Blind-ML starting kernel: the workload's Gaussian 20 % too wide."""
+    if name == "beads-64x64x32":
         return gaussian_kernel(shape, 1.5 * 1.2, 3.0 * 1.2)
     return gaussian_kernel(shape, 2.0 * 1.2, 4.5 * 1.2, side_lobe=0.1)

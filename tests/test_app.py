@@ -43,7 +43,7 @@ def test_cast_helper():
 
 
 def _write_inputs(tmp_path, scale):
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     img = np.round(case.obs * scale)
     img = np.clip(img, 0, 65535).astype(np.uint16)
     k = synth.gaussian_kernel((16, 16, 16), 1.5, 3.0)

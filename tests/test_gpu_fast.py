@@ -57,9 +57,9 @@ def _case(ishape, kshape, seed):
 
 
 SHAPES = [
-    ((32, 64, 64), (32, 64, 64)),      # config 1: work 64 x 128 x 128
+    ((32, 64, 64), (32, 64, 64)),      # the 64x64x32 beads workload
     ((30, 50, 61), (34, 78, 67)),      # odd image dims, work 64 x 128 x 128
-    ((64, 128, 128), (64, 128, 128)),  # work 128 x 256 x 256 (config-5 patch)
+    ((64, 128, 128), (64, 128, 128)),  # a training patch of the warm-start network
     ((48, 96, 96), (48, 96, 96)),      # work 96 x 192 x 192: radix 3
     ((32, 192, 64), (32, 192, 64)),    # work 64 x 384 x 128: radix 6 in y
     ((40, 70, 150), (56, 122, 42)),    # work 96 x 192 x 192 from odd dims
@@ -129,21 +129,21 @@ def test_fast_on_shapes_outside_its_sizes(clarity_gpu, ishape, kshape):
     assert err <= 1e-4
 
 
-def test_fast_matches_generic_100_iterations_cfg1(clarity_gpu):
+def test_fast_matches_generic_100_iterations_beads64(clarity_gpu):
     api = clarity_gpu
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     a, ea = run_plan(api, case.obs, case.psf, 100)
     b, eb = run_plan(api, case.obs, case.psf, 100, generic=True)
     assert (ea, eb) == ("fast", "generic")
     ref = oracle.ml(case.obs, case.psf, 100)
-    print(f"[fast] cfg1 100 it: fast {rel_l2(a, ref):.3e} generic {rel_l2(b, ref):.3e}")
+    print(f"[fast] beads 64x64x32 100 it: fast {rel_l2(a, ref):.3e} generic {rel_l2(b, ref):.3e}")
     assert rel_l2(a, ref) <= 1e-4
     assert rel_l2(b, ref) <= 1e-4
 
 
 def test_fast_initial_estimate(clarity_gpu):
     api = clarity_gpu
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     init = np.full(case.obs.shape, float(case.obs.mean()), np.float32)
     got, engine = run_plan(api, case.obs, case.psf, 5, initial=init)
     assert engine == "fast"
@@ -173,7 +173,7 @@ def test_fast_headline_shape_invariants(clarity_gpu):
 
 def test_fast_timing_api(clarity_gpu):
     api = clarity_gpu
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     plan = api.RLPlan(case.obs.shape, case.psf.shape)
     plan.set_inputs(case.obs, case.psf)
     kinds = plan.describe()
@@ -199,7 +199,7 @@ def test_fused_x_boundary_matches_split(clarity_gpu, tune, exact):
     bit for bit; with them (the default) x_fwd and x_last may contract the
     butterflies of the known-zero half differently, so to float rounding."""
     api = clarity_gpu
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     if tune is not None:
         os.environ["CLARITY_FAST_TUNE"] = tune
     try:
@@ -245,7 +245,7 @@ def test_graph_replay_matches_eager(clarity_gpu):
     results to eager launches, replay after replay, with eager work (the
     estimate copy-out and a host sync) between the replays."""
     api = clarity_gpu
-    case = synth.make_config(1)
+    case = synth.make_workload("beads-64x64x32")
     os.environ["CLARITY_NO_GRAPH"] = "1"
     try:
         eager = api.RLPlan(case.obs.shape, case.psf.shape)

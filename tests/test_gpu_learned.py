@@ -1,4 +1,4 @@
-"""Learned warm start on the GPU, judged by plain RL: the committed network
+"""Learned warm start on the GPU, measured against plain RL: the committed network
 (clarity_amd/data/unet_warmstart.npz, trained by clarity_amd.learned on
 seeded patches 0..1999, validated on 2000..2031) on held-out patches, and
 the gate of clarity_amd.learned.deconvolve between the learned path and
@@ -50,12 +50,12 @@ def test_gain_over_plain_rl(clarity_gpu, trained):
         plain20 = plan.estimate()
         plan.iterate(20)
         plain40 = plan.estimate()
-        plan.iterate(L.JUDGE_ITERATIONS - 40)
-        judge = plan.estimate()
+        plan.iterate(L.YARDSTICK_ITERATIONS - 40)
+        yard = plan.estimate()
         plan.reset(net.predict(p.obs, p.sigma_xy, p.sigma_z))
         plan.iterate(L.REFINE_ITERATIONS)
         learned = plan.estimate()
-        rows.append((rel_l2(learned, judge), rel_l2(plain20, judge), rel_l2(plain40, judge)))
+        rows.append((rel_l2(learned, yard), rel_l2(plain20, yard), rel_l2(plain40, yard)))
         print(f"[learned]   rel L2 to RL-200: learned {rows[-1][0]:.4f} "
               f"plain-20 {rows[-1][1]:.4f} plain-40 {rows[-1][2]:.4f}", flush=True)
     plan.close()
@@ -71,7 +71,7 @@ def test_gate_falls_back_to_plain_rl(clarity_gpu, trained):
     plan = api.RLPlan(L.PATCH, L.PATCH)
     plan.set_inputs(p.obs, p.psf)
     plan.reset()
-    plan.iterate(L.JUDGE_ITERATIONS)
+    plan.iterate(L.YARDSTICK_ITERATIONS)
     plain = plan.estimate()
     plan.reset(net.predict(p.obs, p.sigma_xy, p.sigma_z))
     plan.iterate(L.REFINE_ITERATIONS)

@@ -28,8 +28,8 @@ def rel_l2(a, b):
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    return synth.make_config(1)
+def small():
+    return synth.make_workload("beads-64x64x32")
 
 
 @pytest.fixture(autouse=True)
@@ -127,13 +127,13 @@ def test_clip_bit_exact(clarity_gpu):
 
 # ---------------------------------------------------------------- drivers
 
-def test_convolve_vs_oracle_cfg1(clarity_gpu, cfg1):
+def test_convolve_vs_oracle_small(clarity_gpu, small):
     api = clarity_gpu
-    got = api.Convolve(cfg1.obj, cfg1.psf)
-    ref = oracle.convolve(cfg1.obj, cfg1.psf)
+    got = api.Convolve(small.obj, small.psf)
+    ref = oracle.convolve(small.obj, small.psf)
     assert rel_l2(got, ref) <= 1e-5
     # the oracle itself against numpy float64 (independent FFT)
-    assert rel_l2(ref, synth.np_convolve(cfg1.obj, cfg1.psf)) <= 1e-6
+    assert rel_l2(ref, synth.np_convolve(small.obj, small.psf)) <= 1e-6
 
 
 @pytest.mark.parametrize("ishape,kshape", [
@@ -158,10 +158,10 @@ def test_convolve_delta_identity(clarity_gpu):
     assert rel_l2(api.Convolve(a, k), a) <= 1e-6
 
 
-def test_wiener_vs_oracle(clarity_gpu, cfg1):
+def test_wiener_vs_oracle(clarity_gpu, small):
     api = clarity_gpu
-    got = api.WienerDeconvolve(cfg1.obs, cfg1.psf, 1e-3)
-    ref = oracle.wiener(cfg1.obs, cfg1.psf, 1e-3)
+    got = api.WienerDeconvolve(small.obs, small.psf, 1e-3)
+    ref = oracle.wiener(small.obs, small.psf, 1e-3)
     assert rel_l2(got, ref) <= 1e-5
 
 
@@ -175,51 +175,51 @@ def test_wiener_delta_kernel(clarity_gpu):
 
 
 @pytest.mark.parametrize("iters", [1, 10])
-def test_ml_vs_oracle_cfg1(clarity_gpu, cfg1, iters):
+def test_ml_vs_oracle_small(clarity_gpu, small, iters):
     api = clarity_gpu
-    print(f"[ml] cfg1 iterations={iters}", flush=True)
-    got = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, iters)
-    ref = oracle.ml(cfg1.obs, cfg1.psf, iters)
+    print(f"[ml] small iterations={iters}", flush=True)
+    got = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, iters)
+    ref = oracle.ml(small.obs, small.psf, iters)
     err = rel_l2(got, ref)
     with oracle.reduce_mode(oracle.REDUCE_SERIAL_F32):
-        ref32 = oracle.ml(cfg1.obs, cfg1.psf, iters)
+        ref32 = oracle.ml(small.obs, small.psf, iters)
     print(f"[ml] rel_l2 vs fp64-oracle {err:.3e}; vs serial-f32 {rel_l2(got, ref32):.3e}")
     assert err <= (1e-5 if iters == 1 else 1e-4)
 
 
-def test_ml_energy_and_support(clarity_gpu, cfg1):
+def test_ml_energy_and_support(clarity_gpu, small):
     api = clarity_gpu
-    out = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 5)
+    out = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 5)
     assert np.all(out >= 0)
-    assert abs(out.astype(np.float64).sum() / cfg1.obs.astype(np.float64).sum() - 1) < 1e-5
+    assert abs(out.astype(np.float64).sum() / small.obs.astype(np.float64).sum() - 1) < 1e-5
 
 
-def test_ml_scale_invariance(clarity_gpu, cfg1):
+def test_ml_scale_invariance(clarity_gpu, small):
     api = clarity_gpu
-    a = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 3)
-    b = api.MaximumLikelihoodDeconvolve(cfg1.obs * 4.0, cfg1.psf, 3)
+    a = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 3)
+    b = api.MaximumLikelihoodDeconvolve(small.obs * 4.0, small.psf, 3)
     assert rel_l2(b, 4.0 * a) <= 1e-5
 
 
-def test_ml_zero_iterations_and_errors(clarity_gpu, cfg1):
+def test_ml_zero_iterations_and_errors(clarity_gpu, small):
     api = clarity_gpu
-    out = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 0)
-    assert np.array_equal(out, cfg1.obs)
+    out = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 0)
+    assert np.array_equal(out, small.obs)
     with pytest.raises(api.ClarityError) as e:
-        api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, -1)
+        api.MaximumLikelihoodDeconvolve(small.obs, small.psf, -1)
     assert e.value.code == api.INVALID_ARGUMENT
 
 
-def test_ml_from_initial(clarity_gpu, cfg1):
+def test_ml_from_initial(clarity_gpu, small):
     api = clarity_gpu
-    init = np.full(cfg1.obs.shape, float(cfg1.obs.mean()), np.float32)
-    got = api.MaximumLikelihoodDeconvolveFrom(cfg1.obs, cfg1.psf, init, 4)
-    ref = oracle.ml_ex(cfg1.obs, cfg1.psf, 4, initial=init)
+    init = np.full(small.obs.shape, float(small.obs.mean()), np.float32)
+    got = api.MaximumLikelihoodDeconvolveFrom(small.obs, small.psf, init, 4)
+    ref = oracle.ml_ex(small.obs, small.psf, 4, initial=init)
     assert rel_l2(got, ref) <= 1e-4
     # resuming: 3 + 2 iterations == 5 iterations
-    a = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 3)
-    b = api.MaximumLikelihoodDeconvolveFrom(cfg1.obs, cfg1.psf, a, 2)
-    c = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 5)
+    a = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 3)
+    b = api.MaximumLikelihoodDeconvolveFrom(small.obs, small.psf, a, 2)
+    c = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 5)
     assert rel_l2(b, c) <= 1e-5
 
 
@@ -238,23 +238,23 @@ def test_ml_mixed_radix_reference_geometry(clarity_gpu):
     assert rel_l2(got, ref) <= 1e-4
 
 
-def test_jvc_vs_oracle(clarity_gpu, cfg1):
+def test_jvc_vs_oracle(clarity_gpu, small):
     api = clarity_gpu
     for iters in (0, 1, 5):
-        got = api.JansenVanCittertDeconvolve(cfg1.obs, cfg1.psf, iters)
-        ref = oracle.jvc(cfg1.obs, cfg1.psf, iters)
+        got = api.JansenVanCittertDeconvolve(small.obs, small.psf, iters)
+        ref = oracle.jvc(small.obs, small.psf, iters)
         assert rel_l2(got, ref) <= 1e-4, iters
 
 
-def test_blind_as_written_vs_oracle(clarity_gpu, cfg1):
+def test_blind_as_written_vs_oracle(clarity_gpu, small):
     api = clarity_gpu
-    k0 = synth.blind_initial_kernel(1, cfg1.psf.shape)
-    got = api.BlindMaximumLikelihoodDeconvolve(cfg1.obs, k0, 3)
-    ref = oracle.blind_ml(cfg1.obs, k0, 3)
+    k0 = synth.blind_initial_kernel("beads-64x64x32",small.psf.shape)
+    got = api.BlindMaximumLikelihoodDeconvolve(small.obs, k0, 3)
+    ref = oracle.blind_ml(small.obs, k0, 3)
     assert rel_l2(got, ref) <= 1e-4
 
 
-def test_blind_rl_vs_oracle(clarity_gpu, cfg1):
+def test_blind_rl_vs_oracle(clarity_gpu, small):
     """Alternating blind RL amplifies float32 rounding: on this case the
     oracle run with a float32 FFT (the FFTW3 API, as the reference computes)
     differs from the float64-FFT oracle by 1.05e-3 (image) and 6.9e-3
@@ -264,12 +264,12 @@ def test_blind_rl_vs_oracle(clarity_gpu, cfg1):
     spread, measured here when the FFTW3 API is present (else the values
     above)."""
     api = clarity_gpu
-    k0 = synth.blind_initial_kernel(1, cfg1.psf.shape)
-    o, h = api.BlindRL(cfg1.obs, k0, 2, 2)
-    ro, rh = oracle.blind_rl(cfg1.obs, k0, 2, 2)
+    k0 = synth.blind_initial_kernel("beads-64x64x32",small.psf.shape)
+    o, h = api.BlindRL(small.obs, k0, 2, 2)
+    ro, rh = oracle.blind_rl(small.obs, k0, 2, 2)
     spread_o, spread_h = 1.05e-3, 6.9e-3
     if oracle.set_fft_backend(oracle.FFT_FFTW_API):
-        fo, fh = oracle.blind_rl(cfg1.obs, k0, 2, 2)
+        fo, fh = oracle.blind_rl(small.obs, k0, 2, 2)
         oracle.set_fft_backend(oracle.FFT_OWN)
         spread_o, spread_h = rel_l2(fo, ro), rel_l2(fh, rh)
     eo, eh = rel_l2(o, ro), rel_l2(h, rh)
@@ -302,19 +302,19 @@ def test_stubs_write_zeros(clarity_gpu):
     assert rc == api.SUCCESS and not out.any()
 
 
-def test_device_oom_injection(clarity_gpu, cfg1):
+def test_device_oom_injection(clarity_gpu, small):
     api = clarity_gpu
     for every in (1, 3):
         api.lib().ClarityEx_InjectDeviceOOM(every)
         try:
             with pytest.raises(api.ClarityError) as e:
-                api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 2)
+                api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 2)
             assert e.value.code == api.DEVICE_OUT_OF_MEMORY
         finally:
             api.lib().ClarityEx_InjectDeviceOOM(0)
     # the library recovers: same call succeeds afterwards
-    out = api.MaximumLikelihoodDeconvolve(cfg1.obs, cfg1.psf, 2)
-    assert rel_l2(out, oracle.ml(cfg1.obs, cfg1.psf, 2)) <= 1e-4
+    out = api.MaximumLikelihoodDeconvolve(small.obs, small.psf, 2)
+    assert rel_l2(out, oracle.ml(small.obs, small.psf, 2)) <= 1e-4
 
 
 def test_register_refcount(clarity_gpu):
@@ -340,7 +340,7 @@ def test_calls_without_register(tmp_path):
         "import numpy as np\n"
         "import oracle\n"
         "from clarity_amd import api, synth\n"
-        "c = synth.make_config(1)\n"
+        "c = synth.make_workload('beads-64x64x32')\n"
         "out = api.MaximumLikelihoodDeconvolve(c.obs, c.psf, 3)\n"
         "ref = oracle.ml(c.obs, c.psf, 3)\n"
         "e = np.linalg.norm(out.astype(np.float64) - ref) / np.linalg.norm(ref)\n"

@@ -36,7 +36,7 @@ def test_patch_kernel_is_symmetric():
 
 
 def _card(err, gap):
-    return {"sigma_xy": [1.2, 2.5], "sigma_z": [2.5, 5.0], "judge_iterations": 200,
+    return {"sigma_xy": [1.2, 2.5], "sigma_z": [2.5, 5.0], "yardstick_iterations": 200,
             "refine_iterations": 20,
             "evaluation": {"learned_rel_l2": err, "learned_idiv_rel_gap": gap}}
 
@@ -47,7 +47,7 @@ def test_gate():
     assert not L.accepts(_card(5e-4, 5e-3), 2.0, 4.5)           # I-divergence gap
     assert not L.accepts(_card(5e-4, 1e-4), 3.0, 4.5)           # widths out of range
     assert not L.accepts(_card(5e-4, 1e-4), 2.0, 5.5)
-    assert not L.accepts({"sigma_xy": [1, 3], "sigma_z": [1, 6]}, 2.0, 4.5)  # never judged
+    assert not L.accepts({"sigma_xy": [1, 3], "sigma_z": [1, 6]}, 2.0, 4.5)  # never measured
     assert L.accepts(_card(0.1, 0.02), 2.0, 4.5, tolerance=0.15, idiv_tolerance=0.05)
 
 
@@ -66,7 +66,7 @@ def test_card_round_trip(tmp_path):
 
 def test_committed_weights_match_network():
     """The committed weights have the network's parameter count and a card
-    from the judge (plain RL) on the 512x512x128 benchmark volume."""
+    from the yardstick (plain RL) on the 512x512x128 benchmark volume."""
     import os
     if not os.path.exists(L.WEIGHTS):
         pytest.skip("no trained weights committed")
@@ -74,6 +74,6 @@ def test_committed_weights_match_network():
     params, card = L.load()
     assert params.size == U.layout()[1] and np.all(np.isfinite(params))
     ev = card["evaluation"]
-    assert ev["volume"] == [128, 512, 512] and ev["judge_iterations"] == 200
+    assert ev["volume"] == [128, 512, 512] and ev["yardstick_iterations"] == 200
     assert ev["accepted"] == (ev["learned_rel_l2"] <= L.ACCEPT_REL_L2 and
                               ev["learned_idiv_rel_gap"] <= L.ACCEPT_IDIV)
