@@ -195,6 +195,19 @@ def suite(api, threads):
     cb = cpu_time(lambda: oracle.blind_rl(obs, k0, 1, 1))
     emit("blind rl 4 outer x 5 inner 256x256x64", g, 20 * cb, 0,
          "cpu: 20 x the cost of a 1 x 1 call")
+    # the 512x512x128 volume's blind run: 20 outer x 5 inner, starting from
+    # the kernel's Gaussian 20 % too wide
+    big = (128, 512, 512)
+    rng = np.random.default_rng(3)
+    obj_b = synth.beads_and_filaments(big, 16000, 2400, rng)
+    psf_b = synth.gaussian_kernel(big, 2.0, 4.5, side_lobe=0.1)
+    obs_b = synth.poisson(api.Convolve(obj_b, psf_b), rng)
+    k0_b = synth.gaussian_kernel(big, 2.4, 5.4, side_lobe=0.1)
+    g = gpu_time(lambda: api.BlindRL(obs_b, k0_b, 20, 5), reps=1)
+    cb = cpu_time(lambda: oracle.blind_rl(obs_b, k0_b, 1, 1))
+    emit("blind rl 20 outer x 5 inner 512x512x128", g, 100 * cb, 0,
+         "cpu: 100 x the cost of a 1 x 1 call")
+    del obj_b, psf_b, obs_b, k0_b
     # 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
     rng = np.random.default_rng(4)
     imgs, ks = [], []
