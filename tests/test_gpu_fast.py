@@ -84,6 +84,46 @@ def test_fast_vs_oracle(clarity_gpu, ishape, kshape):
         assert err <= tol
 
 
+def _axis_cases():
+    """(image, kernel, expected engine work shape): one axis at a time is
+    sized so that max(k, i + k/2) lands just below each of the engine's
+    lengths along it (the other two axes minimal), so every x, y and z
+    kernel instantiation of the default launch shapes runs once."""
+    small_i, small_k = {0: (8, 4), 1: (16, 8), 2: (16, 8)}, None
+    lengths = {2: [(64, 64, 128), (120, 40, 192), (200, 40, 256), (300, 60, 384),
+                   (440, 60, 512), (600, 100, 768), (900, 60, 1024)],
+               1: [(64, 64, 128), (120, 40, 192), (200, 40, 256), (300, 60, 384),
+                   (440, 60, 512), (600, 100, 768), (900, 60, 1024), (1500, 100, 2048)],
+               0: [(20, 10, 64), (60, 30, 96), (80, 40, 128), (120, 40, 192),
+                   (160, 80, 256), (300, 100, 512)]}
+    minimal = {0: 64, 1: 128, 2: 128}
+    out = []
+    for ax, rows in lengths.items():
+        for i, k, n in rows:
+            ishape = [small_i[a][0] for a in range(3)]
+            kshape = [small_i[a][1] for a in range(3)]
+            ishape[ax], kshape[ax] = i, k
+            work = [minimal[a] for a in range(3)]
+            work[ax] = n
+            out.append((tuple(ishape), tuple(kshape), tuple(work)))
+    return out
+
+
+@pytest.mark.parametrize("ishape,kshape,work", _axis_cases())
+def test_fast_every_axis_length(clarity_gpu, ishape, kshape, work):
+    api = clarity_gpu
+    obs, psf = _case(ishape, kshape, 31)
+    plan = api.RLPlan(obs.shape, psf.shape)
+    assert (plan.engine, plan.work_shape) == ("fast", work)
+    plan.set_inputs(obs, psf)
+    plan.iterate(3)
+    got = plan.estimate()
+    plan.close()
+    err = rel_l2(got, oracle.ml(obs, psf, 3))
+    print(f"[fast] image {ishape} kernel {kshape} on {work}: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
 def test_fast_selected_only_for_supported_shapes(clarity_gpu):
     """The fused engine transforms over the smallest of its sizes that is
     at least max(k, i + k/2) per axis, so shapes whose i + k is no size of
