@@ -43,6 +43,10 @@
 namespace clarity {
 namespace {
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format;
+// no stride, swizzle or index).
+constexpr int kBufWord3 = 0x00020000;
+
 // ---------------------------------------------------------------- butterflies
 
 // cos / sin of 2*pi*j/16
@@ -157,6 +161,106 @@ __device__ __forceinline__ void dft(float2 (&u)[R]) {
   }
 }
 
+// Pruned butterflies for transforms whose input is known to be zero in part
+// or whose output is wanted only in part (the same-size z pass: the
+// spectrum column is zero in its upper half, the kernel column in its
+// middle half, and the inverse is stored for its lower half only).  ZP: 0
+// none, 1 inputs r >= R/2 are zero, 2 inputs R/4 <= r < 3R/4 are zero.
+// KP: 0 every output, 1 outputs r < R/2 only (the others are left unset).
+// Terms known to be zero are left out, never added (x + 0 is not foldable
+// in IEEE arithmetic, -0 + 0 being +0).
+constexpr int kZeroUpper = 1, kZeroMiddle = 2, kKeepLower = 1;
+
+// DFT4 with zero inputs Z (bit n: u[n] = 0) and outputs k < KN only.
+template <int Z, int KN>
+__device__ __forceinline__ void dft4_p(float2 (&u)[4], int sign) {
+  constexpr bool z1 = Z & 2, z2 = Z & 4, z3 = Z & 8;
+  const float2 t0 = z2 ? u[0] : cadd(u[0], u[2]);
+  const float2 t1 = z2 ? u[0] : csub(u[0], u[2]);
+  const float2 t2 = z3 ? u[1] : z1 ? u[3] : cadd(u[1], u[3]);
+  const float2 d = z3 ? u[1] : z1 ? make_float2(-u[3].x, -u[3].y) : csub(u[1], u[3]);
+  const float2 t3 = cmul_si(d, sign);
+  u[0] = cadd(t0, t2);
+  u[1] = cadd(t1, t3);
+  if constexpr (KN > 2) u[2] = csub(t0, t2);
+  if constexpr (KN > 3) u[3] = csub(t1, t3);
+}
+
+template <int SIGN, int ZP, int KP>
+__device__ __forceinline__ void dft16_p(float2 (&u)[16]) {
+  // n = 4*n1 + n2: upper half zero is n1 >= 2, middle half zero n1 in {1, 2}
+  constexpr int Z = ZP == kZeroUpper ? 0xC : ZP == kZeroMiddle ? 0x6 : 0;
+  // k = k1 + 4*k2 < 8 is k2 < 2
+  constexpr int KN = KP == kKeepLower ? 2 : 4;
+  float2 a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+    float2 t[4] = {u[n2], u[4 + n2], u[8 + n2], u[12 + n2]};
+    dft4_p<Z, 4>(t, SIGN);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) a[n2][k1] = t[k1];
+  }
+  a[1][1] = rot16<SIGN, 1>(a[1][1]);
+  a[1][2] = rot16<SIGN, 2>(a[1][2]);
+  a[1][3] = rot16<SIGN, 3>(a[1][3]);
+  a[2][1] = rot16<SIGN, 2>(a[2][1]);
+  a[2][2] = rot16<SIGN, 4>(a[2][2]);
+  a[2][3] = rot16<SIGN, 6>(a[2][3]);
+  a[3][1] = rot16<SIGN, 3>(a[3][1]);
+  a[3][2] = rot16<SIGN, 6>(a[3][2]);
+  a[3][3] = rot16<SIGN, 9>(a[3][3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    float2 t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4_p<0, KN>(t, SIGN);
+#pragma unroll
+    for (int k2 = 0; k2 < KN; k2++) u[k1 + 4 * k2] = t[k2];
+  }
+}
+
+template <int SIGN, int ZP, int KP>
+__device__ __forceinline__ void dft8_p(float2 (&u)[8]) {
+  // even inputs u[0,2,4,6] and odd u[1,3,5,7], each as a DFT4 over m (u[2m
+  // (+1)]): upper half zero is m >= 2, middle half zero m in {1, 2}
+  constexpr int Z = ZP == kZeroUpper ? 0xC : ZP == kZeroMiddle ? 0x6 : 0;
+  constexpr float c = 0.70710678118654752440f;
+  float2 e[4] = {u[0], u[2], u[4], u[6]};
+  float2 o[4] = {u[1], u[3], u[5], u[7]};
+  dft4_p<Z, 4>(e, SIGN);
+  dft4_p<Z, 4>(o, SIGN);
+  constexpr float s = (float)SIGN;
+  const float2 o1 = make_float2(c * (o[1].x - s * o[1].y), c * (o[1].y + s * o[1].x));
+  const float2 o2 = cmul_si(o[2], SIGN);
+  const float2 o3 = make_float2(c * (-o[3].x - s * o[3].y), c * (-o[3].y + s * o[3].x));
+  u[0] = cadd(e[0], o[0]);
+  u[1] = cadd(e[1], o1);
+  u[2] = cadd(e[2], o2);
+  u[3] = cadd(e[3], o3);
+  if constexpr (KP != kKeepLower) {
+    u[4] = csub(e[0], o[0]);
+    u[5] = csub(e[1], o1);
+    u[6] = csub(e[2], o2);
+    u[7] = csub(e[3], o3);
+  }
+}
+
+// dft<R, SIGN> with the patterns applied where the radix has a pruned form
+// (16, 8, and 2 for KP); other radices compute every term.
+template <int R, int SIGN, int ZP, int KP>
+__device__ __forceinline__ void dft_p(float2 (&u)[R]) {
+  if constexpr (ZP == 0 && KP == 0) {
+    dft<R, SIGN>(u);
+  } else if constexpr (R == 16) {
+    dft16_p<SIGN, ZP, KP>(u);
+  } else if constexpr (R == 8) {
+    dft8_p<SIGN, ZP, KP>(u);
+  } else if constexpr (R == 2 && ZP == 0) {
+    u[0] = cadd(u[0], u[1]);
+  } else {
+    dft<R, SIGN>(u);
+  }
+}
+
 template <int SIGN>
 __device__ __forceinline__ float2 tw_load(const float2 *__restrict__ tw, int idx) {
   float2 w = tw[idx];
@@ -253,13 +357,26 @@ using DefaultRadices = typename RadixList<N, E, 1, false>::type;
 // synchronise with its sync(): a wavefront-local barrier when every pencil
 // lies inside one wavefront, else a workgroup barrier (then all threads of
 // the workgroup must call this together).
-template <int N, int E, int P, int SIGN, class L, int R, int... Rest>
-__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
+//
+// fft_pruned: the same with the pruning patterns of dft_p, ZP on the first
+// stage's inputs and KP on the last stage's outputs, stated for v: ZP 1
+// is v[e] = 0 for e >= E/2, ZP 2 for E/4 <= e < 3E/4; KP 1 asks for v[e],
+// e < E/2, only (the rest of v is left unset).  With u[r] = v[b + r*NB]
+// (NB = E/R, b < NB) these are the patterns r >= R/2, R/4 <= r < 3R/4 and
+// r < R/2 of every butterfly of the stage.
+template <int N, int E, int P, int SIGN, int ZP, int KP, class L, int R, int... Rest>
+__device__ __forceinline__ void fft_pruned(float2 (&v)[E], int t, const L &lds,
                                            const float2 *__restrict__ tw,
                                            Radices<R, Rest...>) {
   constexpr int T = N / E;
   constexpr bool LAST = sizeof...(Rest) == 0;
   constexpr int NB = E / R;
+  // patterns that do not fall on whole butterfly inputs are not applied
+  constexpr int ZS = (P == 1 && ((ZP == kZeroUpper && R % 2 == 0) ||
+                                 (ZP == kZeroMiddle && R % 4 == 0)))
+                         ? ZP
+                         : 0;
+  constexpr int KS = (LAST && KP == kKeepLower && R % 2 == 0) ? KP : 0;
   static_assert(E % R == 0, "radix must divide the per-thread count");
   static_assert(!LAST || P * R == N, "radices must multiply to N");
 #pragma unroll
@@ -285,10 +402,10 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
           u[r] = cmul(u[r], tw_load<SIGN>(tw, r * k * (N / (P * R))));
       }
     }
-    dft<R, SIGN>(u);
+    dft_p<R, SIGN, ZS, KS>(u);
     if constexpr (LAST) {
 #pragma unroll
-      for (int r = 0; r < R; r++) v[b + r * NB] = u[r];
+      for (int r = 0; r < (KS ? R / 2 : R); r++) v[b + r * NB] = u[r];
     } else {
       const int base = (i - k) * R + k;
 #pragma unroll
@@ -300,8 +417,14 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = lds(t + T * e);
     lds.sync();
-    fft_stages<N, E, P * R, SIGN, L>(v, t, lds, tw, Radices<Rest...>{});
+    fft_pruned<N, E, P * R, SIGN, 0, KP, L>(v, t, lds, tw, Radices<Rest...>{});
   }
+}
+
+template <int N, int E, int P, int SIGN, class L, class Rad>
+__device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
+                                           const float2 *__restrict__ tw, Rad r) {
+  fft_pruned<N, E, P, SIGN, 0, 0>(v, t, lds, tw, r);
 }
 
 // Radix plan for an N-point transform with E elements per thread: powers of
@@ -367,25 +490,69 @@ __device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y)
 
 // Writes X[0..M] of the TP rows held (as Z = FFT_M(x_even + i x_odd)) in
 // LDS (RowLds<PS> layout, rows kRowStride<M, PS> apart), to S with consecutive
-// rows on consecutive lanes.
+// rows on consecutive lanes.  Thread (qq, k0) = (threadIdx % TP, threadIdx /
+// TP) writes row qq at k = k0 + KS*it, KS = NT/TP, unrolled: a multiple of
+// the pad period, so every LDS and table address is the first one plus a
+// constant.  The stores go through a buffer over plane z of S: entries
+// k > M of the last step and rows past the image are given offsets at or
+// beyond its extent and are dropped, with no branch.
 template <int M, int TP, int NT, int PS>
 __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restrict__ S,
                                              const float2 *twN, const Geom &g, int z,
                                              int y0) {
   constexpr int MP = kRowStride<M, PS>;
-  for (int idx = threadIdx.x; idx < TP * (M + 1); idx += NT) {
-    const int qq = idx % TP;
-    const int k = idx / TP;
-    const int kk = (k == M) ? 0 : k;  // Z is M-periodic
-    const int km = (k == 0) ? 0 : M - k;
-    const float2 zk = lds[qq * MP + padded<PS>(kk)];
-    const float2 zm = lds[qq * MP + padded<PS>(km)];
+  static_assert(NT % TP == 0, "a thread keeps one row");
+  constexpr int KS = NT / TP;
+  constexpr int IT = (M + 1 + KS - 1) / KS;
+  // steps of whole pad periods (every size but 128 points at 16 per thread)
+  constexpr bool WHOLE = KS % (1 << PS) == 0;
+  constexpr int KSP = KS + (KS >> PS);  // then padded<PS>(k + KS) - padded<PS>(k)
+  const int qq = threadIdx.x % TP;
+  const int k0 = threadIdx.x / TP;
+  const float2 *row = lds + qq * MP;
+  const float2 *zk0 = row + padded<PS>(k0);
+  const float2 *zm0 = row + padded<PS>(M - k0);
+  const unsigned extent = (unsigned)g.hx * (unsigned)g.ny * (unsigned)sizeof(float2);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      S + (long long)z * g.hx * g.ny, 0, (int)extent, kBufWord3);
+  const unsigned off =
+      (y0 + qq < g.iy) ? (unsigned)(k0 * g.ny + y0 + qq) * (unsigned)sizeof(float2) : extent;
+  const unsigned stepB = (unsigned)KS * (unsigned)g.ny * (unsigned)sizeof(float2);
+  using B64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0u, 0u, 0));
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    float2 zk, zm, w;
+    if (it == 0) {
+      // k = k0: Z[0]'s partner is Z[0] itself
+      zk = zk0[0];
+      zm = (k0 == 0) ? zk : zm0[0];
+      w = twN[k0];
+    } else if (it + 1 < IT) {
+      if constexpr (WHOLE) {
+        zk = zk0[it * KSP];
+        zm = zm0[-it * KSP];
+      } else {
+        zk = row[padded<PS>(k0 + it * KS)];
+        zm = row[padded<PS>(M - k0 - it * KS)];
+      }
+      w = twN[k0 + it * KS];
+    } else {
+      // last step: k = M (Z is M-periodic, Z[M] = Z[0], partner Z[0]) or
+      // past it (dropped), read at k = M
+      const int k = min(k0 + it * KS, M);
+      zk = row[padded<PS>(k == M ? 0 : k)];
+      zm = row[padded<PS>(M - k)];
+      w = twN[k];
+    }
     const float2 a = make_float2(zk.x + zm.x, zk.y - zm.y);  // Zk + conj(Zm)
     const float2 d = make_float2(zk.x - zm.x, zk.y + zm.y);  // Zk - conj(Zm)
-    const float2 wd = cmul(twN[k], d);                        // w^k d
+    const float2 wd = cmul(w, d);                            // w^k d
     // X = (a - i w^k d) / 2
     const float2 X = make_float2(0.5f * (a.x + wd.y), 0.5f * (a.y - wd.x));
-    if (y0 + qq < g.iy) S[s_index(g, z, k, y0 + qq)] = X;
+    // the whole offset in the per-lane operand, the one the range check
+    // is certain to cover
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, X), rs, off + it * stepB, 0u,
+                                          0);
   }
 }
 
@@ -536,6 +703,18 @@ __device__ __forceinline__ void store_row_pair(float *row, int n0, int ix, float
   }
 }
 
+// a / b for normal b (the guarded ratio has |b| >= 1e-5): the hardware
+// reciprocal (1 ulp) and one Newton correction of the quotient, q + (a - b
+// q) / b with the residual exact in one fma; within about 1 ulp of the
+// correctly rounded quotient in 4 instructions, where IEEE division takes
+// about 10 (scaling, reciprocal, refinement, fix-up for b near the range
+// limits, which the guard excludes).
+__device__ __forceinline__ float quotient(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
+
 // x kernels share one shared-memory block: the TP-row tile, then the M-point
 // and (M+1)-entry twiddle tables.
 // Tile index of this workgroup.  With 16-row tiles a tile spans 128 bytes
@@ -643,11 +822,11 @@ __global__ void __launch_bounds__(TP *(M / E))
     float r0 = 0.f, r1 = 0.f;
     if (valid && in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].x;
-      r0 = (fabsf(s) < kDivGuard) ? 0.f : o[e].x / s;
+      r0 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].x, s);
     }
     if (valid && second_in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].y;
-      r1 = (fabsf(s) < kDivGuard) ? 0.f : o[e].y / s;
+      r1 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].y, s);
     }
     v[e] = make_float2(r0, r1);
   }
@@ -858,7 +1037,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
 template <int N, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, float scale) {
+        const float2 *__restrict__ gtw, Geom g) {
   static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
   constexpr int T = N / E;
   constexpr int NT = TP * T;
@@ -917,16 +1096,16 @@ __global__ void __launch_bounds__(TP *(N / E))
   twr.store(tw);
   __syncthreads();
   const InterLds<TP> L{lds + q};
-  fft_stages<N, E, 1, -1>(v, t, L, tw, typename Plan<N, E>::Rad{});
-  fft_stages<N, E, 1, -1>(h, t, L, tw, typename Plan<N, E>::Rad{});
-  // reference ComplexMultiplyAndScale: scale * (a c - b d, a d + b c)
+  // H: the spectrum column is zero in its upper half, the kernel column in
+  // its middle half, and the inverse is stored for its lower half
+  constexpr int ZV = H ? kZeroUpper : 0, ZH = H ? kZeroMiddle : 0, KO = H ? kKeepLower : 0;
+  fft_pruned<N, E, 1, -1, ZV, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
+  fft_pruned<N, E, 1, -1, ZH, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
+  // reference ComplexMultiplyAndScale (a c - b d, a d + b c), its scale
+  // folded into K
 #pragma unroll
-  for (int e = 0; e < E; e++) {
-    const float2 hv = h[e];
-    const float2 a = v[e];
-    v[e] = make_float2(scale * (a.x * hv.x - a.y * hv.y), scale * (a.x * hv.y + a.y * hv.x));
-  }
-  fft_stages<N, E, 1, +1>(v, t, L, tw, typename Plan<N, E>::Rad{});
+  for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
+  fft_pruned<N, E, 1, +1, 0, KO>(v, t, L, tw, typename Plan<N, E>::Rad{});
   {
     float2 *p = s0;
 #pragma unroll
@@ -940,6 +1119,102 @@ __global__ void __launch_bounds__(TP *(N / E))
   }
 }
 
+// The same-size z pass (iz = kz = N/2) over several column tiles per
+// workgroup, software pipelined.  Three N-point transforms per column cost
+// about as much VALU time as the pass's HBM traffic, and a workgroup of the
+// one-tile kernel leaves HBM idle while it computes.  Here, once the product
+// is formed, the kernel column's registers are free: the next tile's
+// spectrum planes (e < E/2) and kernel planes are loaded into them while
+// the current tile's inverse transform runs.  K holds the kernel's planes
+// z < N/4 then z >= 3N/4, so its elements e < E/2 at a column are those
+// planes in order: element e < E/4 is plane t + T*e, element e >= E/4 is
+// plane t + T*(e + E/2).  Tiles are dealt over the grid (tile = blockIdx.x
+// + i * gridDim.x), so the workgroups resident together still read
+// neighbouring 256-byte segments of each plane.
+template <int N, int E, int TP>
+__global__ void __launch_bounds__(TP *(N / E)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_zp(float2 *__restrict__ S, const float2 *__restrict__ K,
+         const float2 *__restrict__ gtw, Geom g) {
+  static_assert(E % 4 == 0, "kernel planes in whole elements");
+  constexpr int T = N / E;
+  constexpr int NT = TP * T;
+  constexpr int EH = E / 2, EQ = E / 4;
+  __shared__ float2 lds[TP * N];
+  __shared__ float2 tw[N];
+  const int q = threadIdx.x % TP, t = threadIdx.x / TP;
+  const int ntile = g.ny / TP;
+  const int ntot = g.hx * ntile;
+  // Raw buffer accesses: a 32-bit per-lane byte offset (plane t and the
+  // lane's column) plus a uniform offset per element (T*e planes), so each
+  // tile's addresses take one register per lane.  The planes read and
+  // written, z < N/2 of S and all N/2 of K, span under 2^32 bytes, the
+  // buffers' extent (an access beyond it reads zero, not faults; the
+  // launcher takes this kernel only where extent + (E/2 - 1) * stepB, the
+  // largest offset sum, stays below 2^32).  The out-of-range offset of the
+  // last prefetch is in the per-lane operand, so those loads fall outside
+  // whether or not the range check adds the uniform one.
+  const unsigned stepB = (unsigned)T * (unsigned)g.sz * (unsigned)sizeof(float2);
+  const unsigned extent = (unsigned)(N / 2) * (unsigned)g.sz * (unsigned)sizeof(float2);
+  const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc(S, 0, (int)extent, kBufWord3);
+  const __amdgpu_buffer_rsrc_t rK =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2 *>(K), 0, (int)extent, kBufWord3);
+  const unsigned tB = (unsigned)t * (unsigned)g.sz * (unsigned)sizeof(float2);
+  const auto offset = [&](int tile) {
+    const int kx = tile / ntile, y0 = (tile - kx * ntile) * TP;
+    return tB + (unsigned)(kx * g.ny + y0 + q) * (unsigned)sizeof(float2);
+  };
+  const auto ld = [&](__amdgpu_buffer_rsrc_t r, int e, unsigned o) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, o, e * stepB, 0));
+  };
+  const auto st = [&](int e, unsigned o, float2 x) {
+    using B64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rS, 0u, 0u, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, x), rS, o, e * stepB, 0);
+  };
+  TableRegs<NT, N> twr;
+  twr.load(gtw);
+  int tile = blockIdx.x;
+  unsigned off = offset(tile);
+  float2 a[EH], k[EH];  // the tile's spectrum planes and kernel planes
+#pragma unroll
+  for (int e = 0; e < EH; e++) a[e] = ld(rS, e, off);
+#pragma unroll
+  for (int e = 0; e < EH; e++) k[e] = ld(rK, e, off);
+  twr.store(tw);
+  __syncthreads();
+  const InterLds<TP> L{lds + q};
+  for (;;) {
+    float2 v[E], h[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      v[e] = (e < EH) ? a[e] : make_float2(0.f, 0.f);
+      h[e] = (e < EQ) ? k[e] : (e >= 3 * EQ) ? k[e - EH] : make_float2(0.f, 0.f);
+    }
+    fft_pruned<N, E, 1, -1, kZeroUpper, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
+    fft_pruned<N, E, 1, -1, kZeroMiddle, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
+    // reference ComplexMultiplyAndScale, its scale folded into K
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
+    // Next tile's loads, here and not earlier (the scheduler would hoist
+    // them above the product, with both columns still live).  After the
+    // last tile the offset is the buffers' extent: the loads fall outside
+    // them, return zero and fetch nothing, with no branch for the compiler
+    // to sink the product behind.
+    const int next = tile + (int)gridDim.x;  // the same for the whole workgroup
+    const unsigned offn = next < ntot ? offset(next) : extent;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int e = 0; e < EH; e++) a[e] = ld(rS, e, offn);
+#pragma unroll
+    for (int e = 0; e < EH; e++) k[e] = ld(rK, e, offn);
+    fft_pruned<N, E, 1, +1, 0, kKeepLower>(v, t, L, tw, typename Plan<N, E>::Rad{});
+#pragma unroll
+    for (int e = 0; e < EH; e++) st(e, off, v[e]);
+    if (next >= ntot) break;
+    tile = next;
+    off = offn;
+  }
+}
+
 // ------------------------------------------------------------ dispatch
 
 enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KX_LAST_FWD, KNUM };
@@ -949,14 +1224,16 @@ const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod",  "y_inv",
 constexpr int kTPz = 16;  // pencils per z tile: 16 consecutive ky -> 128 B
 
 // Launch shape of the fused kernels; defaults per size, overridable with
-// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zt=16,fx=1,xh=7"
+// CLARITY_FAST_TUNE="fe=8,ft=16,me=8,mt=8,le=8,lt=8,yp=4,ze=16,zt=16,zw=1,fx=1,xh=7"
 // (elements per thread and tile rows of x_fwd, x_mid, x_last; y pencils per
-// workgroup; z elements per thread and tile columns; x_last_fwd chaining;
-// same-size specialisations); any subset, the rest keep their defaults.
+// workgroup; z elements per thread, tile columns and tiles per workgroup of
+// the same-size z pass; x_last_fwd chaining; same-size specialisations); any
+// subset, the rest keep their defaults.
 struct Tune {
   int xe[3] = {0, 0, 0};     // x_fwd, x_mid, x_last: elements per thread
   int xt[3] = {16, 16, 16};  // rows per tile
   int yp = 0, ze = 0, zt = 16;
+  int zw = 1;                // same-size z pass: column tiles per workgroup
   int fx = 1;                // chain x_last into the next x_fwd
   int xh = 7;                // bit j: same-size specialisation of x kernel j
                              // (x_fwd, x_mid, x_last, x_last_fwd); measured
@@ -1108,53 +1385,61 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
 }
 
 template <int N, int E, int TP>
-void launch_z_t(const Geom &g, const Bufs &b, float scale, hipStream_t s) {
-  const unsigned grid = (unsigned)(g.hx * (g.ny / TP));
-  if (2 * g.iz == N && 2 * g.kz == N)
+void launch_z_t(int zw, const Geom &g, const Bufs &b, hipStream_t s) {
+  const int ntot = g.hx * (g.ny / TP);
+  const unsigned grid = (unsigned)ntot;
+  // the pipelined kernel's 32-bit buffer offsets: extent plus N/2 - T
+  // planes (its out-of-range offset plus the largest element offset)
+  const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
+  const bool fits32 = 2 * extent < (1ull << 32);
+  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32)
+    hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
+                       dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
+  else if (2 * g.iz == N && 2 * g.kz == N)
     hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g, scale);
+                       b.K, b.twZ, g);
   else
     hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g, scale);
+                       b.K, b.twZ, g);
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
 // its tile within 128 KB of LDS, and ny is a multiple of the width.
 template <int N, int E>
-void launch_z_e(int tp, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+void launch_z_e(int tp, int zw, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr bool ok32 = 32 * (N / E) <= 1024 && 32 * N * 8 <= 131072;
   constexpr bool ok64 = 64 * (N / E) <= 1024 && 64 * N * 8 <= 131072;
   if constexpr (ok64) {
-    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(g, b, scale, s);
+    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(zw, g, b, s);
   }
   if constexpr (ok32) {
-    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(g, b, scale, s);
+    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(zw, g, b, s);
   }
-  launch_z_t<N, E, kTPz>(g, b, scale, s);
+  launch_z_t<N, E, kTPz>(zw, g, b, s);
 }
 
 template <int N>
-void launch_z(int e, int tp, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+void launch_z(int e, int tp, int zw, const Geom &g, const Bufs &b, hipStream_t s) {
   if constexpr (N % 3 == 0) {
-    launch_z_e<N, 12>(tp, g, b, scale, s);
+    launch_z_e<N, 12>(tp, zw, g, b, s);
   } else if constexpr (N == 64) {
-    launch_z_e<N, 8>(tp, g, b, scale, s);
+    launch_z_e<N, 8>(tp, zw, g, b, s);
   } else {
     if (e == 8)
-      launch_z_e<N, 8>(tp, g, b, scale, s);
+      launch_z_e<N, 8>(tp, zw, g, b, s);
     else
-      launch_z_e<N, 16>(tp, g, b, scale, s);
+      launch_z_e<N, 16>(tp, zw, g, b, s);
   }
 }
 
-void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, float scale, hipStream_t s) {
+void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, hipStream_t s) {
   switch (g.nz) {
-    case 64: launch_z<64>(tu.ze, tu.zt, g, b, scale, s); break;
-    case 96: launch_z<96>(tu.ze, tu.zt, g, b, scale, s); break;
-    case 128: launch_z<128>(tu.ze, tu.zt, g, b, scale, s); break;
-    case 192: launch_z<192>(tu.ze, tu.zt, g, b, scale, s); break;
-    case 256: launch_z<256>(tu.ze, tu.zt, g, b, scale, s); break;
-    case 512: launch_z<512>(tu.ze, tu.zt, g, b, scale, s); break;
+    case 64: launch_z<64>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 96: launch_z<96>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 128: launch_z<128>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 192: launch_z<192>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 256: launch_z<256>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 512: launch_z<512>(tu.ze, tu.zt, tu.zw, g, b, s); break;
   }
 }
 
@@ -1212,6 +1497,7 @@ Tune pick_tune(const Geom &g) {
       else if (std::sscanf(p, "yp=%d", &v) == 1) t.yp = v;
       else if (std::sscanf(p, "ze=%d", &v) == 1) t.ze = v;
       else if (std::sscanf(p, "zt=%d", &v) == 1) t.zt = v;
+      else if (std::sscanf(p, "zw=%d", &v) == 1) t.zw = v;
       else if (std::sscanf(p, "fx=%d", &v) == 1) t.fx = v;
       else if (std::sscanf(p, "xh=%d", &v) == 1) t.xh = v;
       while (*p && *p != ',') p++;
@@ -1299,6 +1585,9 @@ class FastRL : public RLEngine {
       CL_TRY(pad.alloc(sizeof(float) * (size_t)pd.voxels()));
       const int kshift[3] = {-w_.kd.x / 2, -w_.kd.y / 2, -w_.kd.z / 2};
       CL_TRY(k_pad_shift(pad.f(), pd, psf_d, w_.kd, kshift, 0.f, s));
+      // the product's 1/(nx ny nz) (the reference's ComplexMultiplyAndScale
+      // scale) folded into K once, so z_mod multiplies by K alone
+      CL_TRY(k_scale(pad.f(), pad.f(), pd.voxels(), inv_volume(g_.nx, g_.ny, g_.nz), s));
       // the iteration's own x and y kernels over every row of the kz
       // planes (the "image" of this transform is all of them)
       Geom go = g_;
@@ -1461,9 +1750,9 @@ class FastRL : public RLEngine {
     CL_HIP(hipStreamSynchronize(s));
     if (desc && cap > 0)
       std::snprintf(desc, (size_t)cap,
-                    "fe=%d,ft=%d,me=%d,mt=%d,le=%d,lt=%d,yp=%d,ze=%d,zt=%d,fx=%d,xh=%d",
+                    "fe=%d,ft=%d,me=%d,mt=%d,le=%d,lt=%d,yp=%d,ze=%d,zt=%d,zw=%d,fx=%d,xh=%d",
                     tu_.xe[0], tu_.xt[0], tu_.xe[1], tu_.xt[1], tu_.xe[2], tu_.xt[2], tu_.yp,
-                    tu_.ze, tu_.zt, tu_.fx, tu_.xh);
+                    tu_.ze, tu_.zt, tu_.zw, tu_.fx, tu_.xh);
     return r;
   }
 
@@ -1552,6 +1841,7 @@ class FastRL : public RLEngine {
         {[](Tune &t, int v) { t.yp = v; }, {2, 4, 8}},
         {[](Tune &t, int v) { t.ze = v; }, {8, 16}},
         {[](Tune &t, int v) { t.zt = v; }, {16, 32, 64}},
+        {[](Tune &t, int v) { t.zw = v; }, {1, 2, 4, 8}},
         {[](Tune &t, int v) { t.fx = v; }, {0, 1}},
     };
     auto same = [](const Tune &a, const Tune &b) {
@@ -1593,7 +1883,6 @@ class FastRL : public RLEngine {
       slot = (int)ev_next_++;
       CL_HIP(hipEventRecord(ev_pool_[slot].first, s));
     }
-    const float scale = inv_volume(g_.nx, g_.ny, g_.nz);
     switch (kind) {
       case KX_FWD:
       case KX_MID:
@@ -1601,7 +1890,7 @@ class FastRL : public RLEngine {
       case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
-      case KZ_MOD: dispatch_z(tu_, g_, b_, scale, s); break;
+      case KZ_MOD: dispatch_z(tu_, g_, b_, s); break;
       case KFINISH:
         CL_TRY(k_finish_sum(b_.parts, nparts_, b_.scal + 1, b_.scal, b_.scal + 2, s));
         break;

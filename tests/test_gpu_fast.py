@@ -261,6 +261,39 @@ def test_fused_x_boundary_matches_split(clarity_gpu, tune, exact):
         assert rel_l2(chained, split) <= 1e-6
 
 
+@pytest.mark.parametrize("ishape,tune", [
+    ((32, 64, 64), "zw=3"),                # z 64, 8 per thread; 520 tiles over 174 groups
+    ((64, 96, 96), "zw=2,ze=16,zt=16"),    # z 128, 16 per thread
+    ((48, 64, 64), "zw=4"),                # z 96, radix 3
+    ((96, 48, 64), "zw=2,zt=32"),          # z 192, radix 3
+    ((128, 64, 64), "zw=4"),               # z 256, 16 per thread, 32-column tiles
+    ((128, 64, 64), "zw=5,ze=8,zt=32"),    # z 256, 8 per thread
+    ((128, 64, 64), "zw=8,ze=16,zt=16"),   # z 256, 16-column tiles
+])
+def test_z_pipelined_matches_one_tile(clarity_gpu, ishape, tune):
+    """The same-size z pass over several column tiles per workgroup (the next
+    tile's planes loaded during the current tile's inverse transform, tile
+    counts that do not divide evenly) gives the one-tile kernel's result to
+    float rounding, and the oracle's."""
+    api = clarity_gpu
+    obs, psf = _case(ishape, ishape, 17)
+    outs = {}
+    for t in ("zw=1", tune):
+        os.environ["CLARITY_FAST_TUNE"] = t
+        try:
+            plan = api.RLPlan(obs.shape, psf.shape)
+        finally:
+            os.environ.pop("CLARITY_FAST_TUNE", None)
+        assert plan.engine == "fast" and plan.work_shape[0] == 2 * ishape[0]
+        plan.set_inputs(obs, psf)
+        plan.reset()
+        plan.iterate(5)
+        outs[t] = plan.estimate()
+        plan.close()
+    assert rel_l2(outs[tune], outs["zw=1"]) <= 1e-6
+    assert rel_l2(outs[tune], oracle.ml(obs, psf, 5)) <= 1e-4
+
+
 def test_batch_uneven_volumes(clarity_gpu):
     """Batched RL over uneven volumes equals per-volume RL (each volume's
     result is the single-volume call's, to float rounding of the engine)."""
