@@ -4,8 +4,10 @@
 // buffers stay on the device from upload to download; sums are float64 on
 // the device and never come back to the host inside a call.
 #include "common.hpp"
+#include "rl_plan.hpp"
 #include "work.hpp"
 
+#include <cstdlib>
 #include <cstring>
 
 namespace clarity {
@@ -179,6 +181,51 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
   return work_download(w, o.f(), out, s);
 }
 
+// Alternating blind RL on the fused RL engines: the image steps are RL
+// iterations of the kernel-fixed problem (the engine on its alias-free work
+// domain), the kernel steps RL iterations of the image-fixed problem with
+// the kernel as the estimate (make_fast_rl_kernel_step: reference work
+// domain, the correction's spectrum sign, renormalisation to sum 1).  The
+// image and kernel estimates stay in their caller layouts, each engine's
+// estimate the other's blurring input.  *done = false when the shapes are
+// not the fused engines' (different or odd dimensions, unsupported sizes).
+ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kernel,
+                               float *out, float *kernel_out, int outer, int inner,
+                               hipStream_t s, bool *done) {
+  *done = false;
+  if (std::getenv("CLARITY_FORCE_GENERIC")) return CLARITY_SUCCESS;
+  std::unique_ptr<RLEngine> img = make_fast_rl(w);
+  std::unique_ptr<RLEngine> ker = make_fast_rl_kernel_step(w);
+  if (!img || !ker) return CLARITY_SUCCESS;
+  *done = true;
+  CL_TRY(img->allocate());
+  CL_TRY(ker->allocate());
+  DevBuf obs, o, h, parts;
+  CL_TRY(obs.alloc(sizeof(float) * w.IV));
+  CL_TRY(o.alloc(sizeof(float) * w.IV));
+  CL_TRY(h.alloc(sizeof(float) * w.KV));
+  CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
+  CL_HIP(hipMemcpyAsync(obs.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(h.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(o.p, obs.p, sizeof(float) * w.IV, hipMemcpyDeviceToDevice, s));
+  CL_TRY(k_normalize_fp64(h.f(), w.KV, parts.d(), kParts, s));
+  for (int it = 0; it < outer; it++) {
+    CL_TRY(img->set_inputs(obs.f(), h.f(), s));
+    CL_TRY(img->reset(o.f(), s));
+    CL_TRY(img->iterate(inner, s));
+    CL_TRY(img->estimate(o.f(), s));
+    CL_TRY(ker->set_inputs(obs.f(), o.f(), s));
+    CL_TRY(ker->reset(h.f(), s));
+    CL_TRY(ker->iterate(inner, s));
+    CL_TRY(ker->estimate(h.f(), s));
+  }
+  if (kernel_out)
+    CL_HIP(hipMemcpyAsync(kernel_out, h.p, sizeof(float) * w.KV, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
 // Alternating blind RL (see ClarityEx.h).
 ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
                          Clarity_Dim3 kd, float *out, float *kernel_out,
@@ -187,6 +234,9 @@ ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
   Work w;
   CL_TRY(work_geometry(w, in, id, kernel, kd, out));
   hipStream_t s = rt().stream;
+  bool fused = false;
+  CL_TRY(blind_rl_fused(w, in, kernel, out, kernel_out, outer, inner, s, &fused));
+  if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf hft, oft, spec, o, s1, s2, parts, scal;
   CL_TRY(hft.alloc(sizeof(float2) * w.C));

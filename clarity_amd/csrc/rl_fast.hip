@@ -1037,7 +1037,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
 template <int N, int E, int TP, bool H>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g) {
+        const float2 *__restrict__ gtw, Geom g, int flip) {
   static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
   constexpr int T = N / E;
   constexpr int NT = TP * T;
@@ -1105,6 +1105,17 @@ __global__ void __launch_bounds__(TP *(N / E))
   // folded into K
 #pragma unroll
   for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
+  if (flip) {
+    // the product also times (-1)^(kx + ky + kz), a cyclic shift by half
+    // the work domain on every axis.  After the forward transforms element
+    // e holds z frequency t + T*e, which has t's parity (T is even): one
+    // sign for the lane's whole column
+    const unsigned m = ((kx + y0 + q + t) & 1) ? 0x80000000u : 0u;
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      v[e] = make_float2(__uint_as_float(__float_as_uint(v[e].x) ^ m),
+                         __uint_as_float(__float_as_uint(v[e].y) ^ m));
+  }
   fft_pruned<N, E, 1, +1, 0, KO>(v, t, L, tw, typename Plan<N, E>::Rad{});
   {
     float2 *p = s0;
@@ -1385,61 +1396,61 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
 }
 
 template <int N, int E, int TP>
-void launch_z_t(int zw, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z_t(int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
   const int ntot = g.hx * (g.ny / TP);
   const unsigned grid = (unsigned)ntot;
   // the pipelined kernel's 32-bit buffer offsets: extent plus N/2 - T
   // planes (its out-of-range offset plus the largest element offset)
   const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
   const bool fits32 = 2 * extent < (1ull << 32);
-  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32)
+  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32 && !flip)
     hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
                        dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
   else if (2 * g.iz == N && 2 * g.kz == N)
     hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g);
+                       b.K, b.twZ, g, (int)flip);
   else
     hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g);
+                       b.K, b.twZ, g, (int)flip);
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
 // its tile within 128 KB of LDS, and ny is a multiple of the width.
 template <int N, int E>
-void launch_z_e(int tp, int zw, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z_e(int tp, int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr bool ok32 = 32 * (N / E) <= 1024 && 32 * N * 8 <= 131072;
   constexpr bool ok64 = 64 * (N / E) <= 1024 && 64 * N * 8 <= 131072;
   if constexpr (ok64) {
-    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(zw, g, b, s);
+    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(zw, flip, g, b, s);
   }
   if constexpr (ok32) {
-    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(zw, g, b, s);
+    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(zw, flip, g, b, s);
   }
-  launch_z_t<N, E, kTPz>(zw, g, b, s);
+  launch_z_t<N, E, kTPz>(zw, flip, g, b, s);
 }
 
 template <int N>
-void launch_z(int e, int tp, int zw, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z(int e, int tp, int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
   if constexpr (N % 3 == 0) {
-    launch_z_e<N, 12>(tp, zw, g, b, s);
+    launch_z_e<N, 12>(tp, zw, flip, g, b, s);
   } else if constexpr (N == 64) {
-    launch_z_e<N, 8>(tp, zw, g, b, s);
+    launch_z_e<N, 8>(tp, zw, flip, g, b, s);
   } else {
     if (e == 8)
-      launch_z_e<N, 8>(tp, zw, g, b, s);
+      launch_z_e<N, 8>(tp, zw, flip, g, b, s);
     else
-      launch_z_e<N, 16>(tp, zw, g, b, s);
+      launch_z_e<N, 16>(tp, zw, flip, g, b, s);
   }
 }
 
-void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, hipStream_t s) {
+void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, hipStream_t s, bool flip) {
   switch (g.nz) {
-    case 64: launch_z<64>(tu.ze, tu.zt, tu.zw, g, b, s); break;
-    case 96: launch_z<96>(tu.ze, tu.zt, tu.zw, g, b, s); break;
-    case 128: launch_z<128>(tu.ze, tu.zt, tu.zw, g, b, s); break;
-    case 192: launch_z<192>(tu.ze, tu.zt, tu.zw, g, b, s); break;
-    case 256: launch_z<256>(tu.ze, tu.zt, tu.zw, g, b, s); break;
-    case 512: launch_z<512>(tu.ze, tu.zt, tu.zw, g, b, s); break;
+    case 64: launch_z<64>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 96: launch_z<96>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 128: launch_z<128>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 192: launch_z<192>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 256: launch_z<256>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 512: launch_z<512>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
   }
 }
 
@@ -1545,6 +1556,11 @@ class FastRL : public RLEngine {
     }
   }
 
+  // The kernel half-step of alternating blind RL: the correction
+  // convolution's spectrum times (-1)^(kx + ky + kz), the estimate
+  // renormalised to sum 1 (see make_fast_rl_kernel_step).
+  void make_kernel_step() { flip_ = unit_ = true; }
+
   static bool supports(const Work &w) {
     return w.wd.x % 2 == 0 && x_supported(w.wd.x / 2) && y_supported(w.wd.y) &&
            z_supported(w.wd.z) && w.wd.y % kTPz == 0;
@@ -1602,9 +1618,11 @@ class FastRL : public RLEngine {
       CL_TRY(launch_check());
       CL_HIP(hipStreamSynchronize(s));  // pad is released on return
     }
-    // E_obs: the padded observation's sum (zeros outside the corner)
+    // E_obs: the padded observation's sum (zeros outside the corner); the
+    // kernel step's target is 1 instead (the estimate renormalised to sum 1)
     CL_TRY(k_sum_partials(obs_.f(), w_.IV, parts_.d(), kParts, s));
     CL_TRY(k_finish_sum(parts_.d(), kParts, scal_.f(), nullptr, nullptr, s));
+    if (unit_) CL_TRY(k_fill(scal_.f(), 1, 1.0f, s));
     return CLARITY_SUCCESS;
   }
 
@@ -1671,7 +1689,10 @@ class FastRL : public RLEngine {
   ClarityResult_t enqueue_body(bool chained, hipStream_t s) {
     for (int half = 0; half < 2; half++) {
       CL_TRY(run(KY_FWD, s));
-      CL_TRY(run(KZ_MOD, s));
+      flip_now_ = flip_ && half == 1;
+      const ClarityResult_t r = run(KZ_MOD, s);
+      flip_now_ = false;
+      CL_TRY(r);
       CL_TRY(run(KY_INV, s));
       CL_TRY(run(half == 0 ? KX_MID : chained ? KX_LAST_FWD : KX_LAST, s));
     }
@@ -1890,7 +1911,7 @@ class FastRL : public RLEngine {
       case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
-      case KZ_MOD: dispatch_z(tu_, g_, b_, s); break;
+      case KZ_MOD: dispatch_z(tu_, g_, b_, s, flip_now_); break;
       case KFINISH:
         CL_TRY(k_finish_sum(b_.parts, nparts_, b_.scal + 1, b_.scal, b_.scal + 2, s));
         break;
@@ -1911,6 +1932,9 @@ class FastRL : public RLEngine {
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, K_, parts_, scal_;
+  bool flip_ = false;      // kernel step: the correction's spectrum sign
+  bool unit_ = false;      // kernel step: renormalise to sum 1
+  bool flip_now_ = false;  // the z_mod being enqueued takes the sign
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_next_ = 0;
@@ -1985,6 +2009,37 @@ std::unique_ptr<RLEngine> make_fast_rl(const Work &ref) {
   geom.scale = inv_volume(geom.wd.x, geom.wd.y, geom.wd.z);
   if (!FastRL::supports(geom)) return nullptr;
   return std::unique_ptr<RLEngine>(new FastRL(geom));
+}
+
+// The kernel half-step of alternating blind RL, h' = h * (O (*) guard(obs /
+// (O (*) h))) then h renormalised to sum 1, as an RL iteration of this
+// engine: the estimate is the kernel in its caller layout (kernelDim, in
+// the corner), the blurring "kernel" input is the image estimate O
+// (imageDim), both convolutions with O and no conjugate, as the generic
+// driver.  Holding h in the corner instead of shifted by -k/2 makes the
+// first convolution the engine's own (O shifted by -i/2); the second needs
+// O shifted by +i/2 instead, which differs by a shift of i.  On the
+// reference work domain with i = k, i.e. n = 2i on every axis, that shift
+// is half the domain: the spectrum times (-1)^(kx + ky + kz), which z_mod
+// applies to the correction.  The reference domain is kept (no smaller
+// alias-free domain): the generic driver's cyclic convolutions wrap around
+// in the correction, and this reproduces them.  nullptr unless image and
+// kernel have the same even dimensions and the domain is supported.
+std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &ref) {
+  const Dims &i = ref.id, &k = ref.kd;
+  if (i.x != k.x || i.y != k.y || i.z != k.z) return nullptr;
+  if (k.x % 2 || k.y % 2 || k.z % 2) return nullptr;
+  if (ref.wd.x != 2 * k.x || ref.wd.y != 2 * k.y || ref.wd.z != 2 * k.z) return nullptr;
+  Work geom;
+  geom.set_geometry(ref);
+  geom.wd = ref.wd;
+  geom.V = geom.wd.voxels();
+  geom.C = (long long)geom.wd.z * geom.wd.y * (geom.wd.x / 2 + 1);
+  geom.scale = inv_volume(geom.wd.x, geom.wd.y, geom.wd.z);
+  if (!FastRL::supports(geom)) return nullptr;
+  auto *e = new FastRL(geom);
+  e->make_kernel_step();
+  return std::unique_ptr<RLEngine>(e);
 }
 
 }  // namespace clarity

@@ -55,5 +55,9 @@ class RLEngine {
 std::unique_ptr<RLEngine> make_generic_rl(const Work &geom);
 // nullptr when the shape is not one the fused pipeline handles.
 std::unique_ptr<RLEngine> make_fast_rl(const Work &geom);
+// The kernel half-step of alternating blind RL on the fused pipeline
+// (estimate = the kernel, blurring input = the image estimate; same even
+// image and kernel dimensions, reference work domain); nullptr otherwise.
+std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &geom);
 
 }  // namespace clarity
