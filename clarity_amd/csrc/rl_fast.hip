@@ -493,9 +493,11 @@ __device__ __forceinline__ long long s_index(const Geom &g, int z, int k, int y)
 // rows on consecutive lanes.  Thread (qq, k0) = (threadIdx % TP, threadIdx /
 // TP) writes row qq at k = k0 + KS*it, KS = NT/TP, unrolled: a multiple of
 // the pad period, so every LDS and table address is the first one plus a
-// constant.  The stores go through a buffer over plane z of S: entries
-// k > M of the last step and rows past the image are given offsets at or
-// beyond its extent and are dropped, with no branch.
+// constant.  Stores address plane z of S by 32-bit byte offsets: entries
+// k > M of the last step and rows past the image have offsets at or beyond
+// the plane's extent and are not stored (global stores under that
+// predicate measured faster than raw buffer stores, which drop such
+// entries by themselves, for the 64-byte spans of 8-row tiles).
 template <int M, int TP, int NT, int PS>
 __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restrict__ S,
                                              const float2 *twN, const Geom &g, int z,
@@ -513,12 +515,10 @@ __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restri
   const float2 *zk0 = row + padded<PS>(k0);
   const float2 *zm0 = row + padded<PS>(M - k0);
   const unsigned extent = (unsigned)g.hx * (unsigned)g.ny * (unsigned)sizeof(float2);
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-      S + (long long)z * g.hx * g.ny, 0, (int)extent, kBufWord3);
+  char *const plane = reinterpret_cast<char *>(S + (long long)z * g.hx * g.ny);
   const unsigned off =
       (y0 + qq < g.iy) ? (unsigned)(k0 * g.ny + y0 + qq) * (unsigned)sizeof(float2) : extent;
   const unsigned stepB = (unsigned)KS * (unsigned)g.ny * (unsigned)sizeof(float2);
-  using B64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0u, 0u, 0));
 #pragma unroll
   for (int it = 0; it < IT; it++) {
     float2 zk, zm, w;
@@ -549,10 +549,8 @@ __device__ __forceinline__ void unpack_store(const float2 *lds, float2 *__restri
     const float2 wd = cmul(w, d);                            // w^k d
     // X = (a - i w^k d) / 2
     const float2 X = make_float2(0.5f * (a.x + wd.y), 0.5f * (a.y - wd.x));
-    // the whole offset in the per-lane operand, the one the range check
-    // is certain to cover
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B64, X), rs, off + it * stepB, 0u,
-                                          0);
+    const unsigned o = off + it * stepB;
+    if (o < extent) *reinterpret_cast<float2 *>(plane + o) = X;
   }
 }
 

@@ -254,75 +254,70 @@ def test_blind_as_written_vs_oracle(clarity_gpu, small):
     assert rel_l2(got, ref) <= 1e-4
 
 
+def _blind_case(name):
+    if name == "beads64":
+        shape, case = (32, 64, 64), synth.make_workload("beads-64x64x32")
+    else:  # 96x64x48: kernel-step domain 96 x 128 x 192, radix 3 along x and z
+        shape = (48, 64, 96)
+        case = synth.make_workload("beads-64x64x32", seed=7, shape=shape)
+    return case.obs, synth.blind_initial_kernel("beads-64x64x32", shape)
+
+
+# float32-FFT spread of the oracle, (image, kernel) relative L2 between its
+# float64 FFT and MKL's FFTW3 API, measured on the CPU; used where the API
+# is absent, else measured in the test
+_BLIND_SPREAD = {
+    ("beads64", (1, 2)): (1.6e-7, 6.0e-4), ("beads64", (2, 1)): (2.3e-4, 7.4e-4),
+    ("beads64", (2, 2)): (1.05e-3, 6.9e-3), ("96x64x48", (1, 2)): (1.4e-7, 3.5e-4),
+    ("96x64x48", (2, 1)): (4.6e-4, 1.7e-3), ("96x64x48", (2, 2)): (2.3e-3, 1.8e-3),
+}
+_blind_ref = {}
+
+
 @pytest.mark.parametrize("engine", ["fused", "generic"])
-def test_blind_rl_vs_oracle(clarity_gpu, small, engine):
-    """Alternating blind RL amplifies float32 rounding: on this case the
-    oracle run with a float32 FFT (the FFTW3 API, as the reference computes)
-    differs from the float64-FFT oracle by 1.05e-3 (image) and 6.9e-3
-    (kernel) after 2 x 2 iterations with MKL on one host and by 2.7e-4 and
-    3.2e-3 on another: two float32 FFTs of the same library already differ
-    by 2x.  The bound is the larger of 1e-4 and four times that float32-FFT
-    spread, measured here when the FFTW3 API is present (else the values
-    above).  Same-size kernels of even dimensions run on the fused engines
-    (image steps on the RL engine, kernel steps on its kernel-step form);
-    CLARITY_FORCE_GENERIC runs the generic driver on the same case."""
+@pytest.mark.parametrize("sched", [(1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("name", ["beads64", "96x64x48"])
+def test_blind_rl_vs_oracle(clarity_gpu, name, sched, engine):
+    """Alternating blind RL (outer x inner) against the float64-FFT oracle,
+    on the fused engines (same-size even shapes: image steps on the RL
+    engine, kernel steps on its kernel-step form) and on the generic driver
+    (CLARITY_FORCE_GENERIC).  The kernel step is ill-conditioned in float32:
+    it keeps the small deviation of O (*) r from its mean, so the oracle's
+    own float32 FFT (the FFTW3 API, as the reference computes) already
+    differs from its float64 FFT by 6e-4 in the kernel after one outer
+    iteration of 2 kernel steps, 7e-3 after 2 x 2, and the two grow
+    chaotically beyond (0.3 at 3 x 2 on 96x64x48).  The bound is a
+    multiple of that spread: 4x, and 8x in the chaotic 2 x 2 regime, where
+    two float32 implementations' distance is a random draw of that scale
+    (two float32 FFTs of one library already differ by 2x from host to
+    host)."""
     api = clarity_gpu
-    k0 = synth.blind_initial_kernel("beads-64x64x32",small.psf.shape)
+    obs, k0 = _blind_case(name)
     if engine == "generic":
         os.environ["CLARITY_FORCE_GENERIC"] = "1"
     try:
-        o, h = api.BlindRL(small.obs, k0, 2, 2)
+        o, h = api.BlindRL(obs, k0, *sched)
     finally:
         os.environ.pop("CLARITY_FORCE_GENERIC", None)
-    ro, rh = oracle.blind_rl(small.obs, k0, 2, 2)
-    spread_o, spread_h = 1.05e-3, 6.9e-3
-    if oracle.set_fft_backend(oracle.FFT_FFTW_API):
-        fo, fh = oracle.blind_rl(small.obs, k0, 2, 2)
-        oracle.set_fft_backend(oracle.FFT_OWN)
-        spread_o, spread_h = rel_l2(fo, ro), rel_l2(fh, rh)
+    key = (name, sched)
+    if key not in _blind_ref:
+        ro, rh = oracle.blind_rl(obs, k0, *sched)
+        spread = _BLIND_SPREAD[key]
+        if oracle.set_fft_backend(oracle.FFT_FFTW_API):
+            fo, fh = oracle.blind_rl(obs, k0, *sched)
+            oracle.set_fft_backend(oracle.FFT_OWN)
+            spread = (rel_l2(fo, ro), rel_l2(fh, rh))
+        _blind_ref[key] = (ro, rh, spread)
+    ro, rh, (spread_o, spread_h) = _blind_ref[key]
+    f = 8 if sched == (2, 2) else 4
     eo, eh = rel_l2(o, ro), rel_l2(h, rh)
-    print(f"[blind] gpu err o {eo:.2e} h {eh:.2e}; f32-FFT spread o {spread_o:.2e} "
-          f"h {spread_h:.2e}")
-    assert eo <= max(1e-4, 4 * spread_o)
-    assert eh <= max(1e-4, 4 * spread_h)
+    print(f"[blind] {name} {sched[0]}x{sched[1]} {engine}: err o {eo:.2e} h {eh:.2e}; "
+          f"f32-FFT spread o {spread_o:.2e} h {spread_h:.2e}")
+    assert eo <= max(1e-4, f * spread_o)
+    assert eh <= max(1e-4, f * spread_h)
     assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
     # h * (O (*) r) is non-negative up to the float32 roundoff of a
     # near-zero convolution value
-    assert h.min() >= -1e-6 * h.max()
-
-
-def test_blind_rl_fused_matches_generic(clarity_gpu):
-    """Fused and generic blind RL on a non-cubic same-size case (kernel-step
-    domain 96 x 128 x 192: radix 3 along x and z), each against the oracle
-    after 2 x 2 iterations, to four times the float32-FFT spread of the
-    oracle on the case (2.3e-3 image, 1.8e-3 kernel with MKL's FFTW3 API).
-    Two outer iterations, because alternating blind RL amplifies rounding
-    chaotically beyond that (at 3 x 2 the two oracles already differ by
-    0.3); the kernel stays a normalised, non-negative kernel."""
-    api = clarity_gpu
-    shape = (48, 64, 96)
-    case = synth.make_workload("beads-64x64x32", seed=7, shape=shape)
-    k0 = synth.blind_initial_kernel("beads-64x64x32", shape)
-    o, h = api.BlindRL(case.obs, k0, 2, 2)
-    os.environ["CLARITY_FORCE_GENERIC"] = "1"
-    try:
-        go, gh = api.BlindRL(case.obs, k0, 2, 2)
-    finally:
-        os.environ.pop("CLARITY_FORCE_GENERIC", None)
-    ro, rh = oracle.blind_rl(case.obs, k0, 2, 2)
-    spread_o, spread_h = 2.3e-3, 1.8e-3
-    if oracle.set_fft_backend(oracle.FFT_FFTW_API):
-        fo, fh = oracle.blind_rl(case.obs, k0, 2, 2)
-        oracle.set_fft_backend(oracle.FFT_OWN)
-        spread_o, spread_h = rel_l2(fo, ro), rel_l2(fh, rh)
-    print(f"[blind] 96x64x48 fused vs generic o {rel_l2(o, go):.2e} h {rel_l2(h, gh):.2e}; "
-          f"fused vs oracle o {rel_l2(o, ro):.2e} h {rel_l2(h, rh):.2e}; "
-          f"generic vs oracle o {rel_l2(go, ro):.2e} h {rel_l2(gh, rh):.2e}; "
-          f"f32-FFT spread o {spread_o:.2e} h {spread_h:.2e}")
-    for a, b in ((o, h), (go, gh)):
-        assert rel_l2(a, ro) <= max(1e-4, 4 * spread_o)
-        assert rel_l2(b, rh) <= max(1e-4, 4 * spread_h)
-    assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
     assert h.min() >= -1e-6 * h.max()
 
 
