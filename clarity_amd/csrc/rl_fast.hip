@@ -450,6 +450,25 @@ struct Geom {
   int kz, ka;
 };
 
+// An engine's device buffers and twiddle tables.
+struct Bufs {
+  float *u, *obs, *scal;  // scal: E, S, c
+  float2 *S, *K;          // spectrum; kernel after its x and y transforms
+  double *parts;
+  const float2 *twM, *twN, *twY, *twZ;
+};
+
+// One volume of a batched launch: a kernel instantiation run over several
+// volumes at once reads its volume's geometry and buffers here (the entry
+// of vmap[blockIdx.x]) and works on block blockIdx.x - blk0 of it.
+struct VolTab {
+  Geom g;
+  Bufs b;
+  int blk0;    // the volume's first block in the launch
+  int flip;    // z_mod: product times (-1)^(kx + ky + kz)
+  int nparts;  // finish: partial sums of the volume's last x pass
+};
+
 // Copies a twiddle table into LDS (cooperatively; caller synchronises).
 // A table of N entries moved from global memory to LDS by NT threads in two
 // phases, load() into registers and store() to LDS, so that a kernel can
@@ -722,8 +741,7 @@ __device__ __forceinline__ float quotient(float a, float b) {
 // time, take the two halves, so the line is fetched into that XCD's L2 once.
 // The grid is padded to a multiple of 16; padding blocks get -1.
 template <int TP>
-__device__ __forceinline__ int x_tile(int ntiles) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ int x_tile(int b, int ntiles) {
   if constexpr (TP == 16) {
     return b < ntiles ? b : -1;
   } else {
@@ -741,8 +759,7 @@ struct XSmem {
 };
 
 template <int M, int E, int TP, int HE>
-__global__ void __launch_bounds__(TP *(M / E))
-    k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
+__device__ __forceinline__ void x_fwd_body(int blk, const float *__restrict__ u, float2 *__restrict__ S,
             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
@@ -750,7 +767,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -774,7 +791,19 @@ __global__ void __launch_bounds__(TP *(M / E))
 
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
+    k_x_fwd(const float *__restrict__ u, float2 *__restrict__ S,
+            const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  x_fwd_body<M, E, TP, HE>(blockIdx.x, u, S, gtwM, gtwN, g);
+}
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    kb_x_fwd(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  x_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.u, v.b.S, v.b.twM, v.b.twN, v.g);
+}
+
+template <int M, int E, int TP, int HE>
+__device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, const float *__restrict__ obs,
             const float *__restrict__ cptr, const float2 *__restrict__ gtwM,
             const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
@@ -783,7 +812,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -837,7 +866,20 @@ __global__ void __launch_bounds__(TP *(M / E))
 
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
+    k_x_mid(float2 *__restrict__ S, const float *__restrict__ obs,
+            const float *__restrict__ cptr, const float2 *__restrict__ gtwM,
+            const float2 *__restrict__ gtwN, Geom g) {
+  x_mid_body<M, E, TP, HE>(blockIdx.x, S, obs, cptr, gtwM, gtwN, g);
+}
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    kb_x_mid(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  x_mid_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.obs, v.b.scal + 2, v.b.twM, v.b.twN, v.g);
+}
+
+template <int M, int E, int TP, int HE>
+__device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
              const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
@@ -846,7 +888,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -898,14 +940,27 @@ __global__ void __launch_bounds__(TP *(M / E))
   if (threadIdx.x == 0) partials[tile] = sh[0];
 }
 
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
+             const float *__restrict__ cptr, double *__restrict__ partials,
+             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  x_last_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, g);
+}
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    kb_x_last(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  x_last_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, v.g);
+}
+
 // x_last of one iteration fused with x_fwd of the next: the updated rows
 // u' = (c*u)*s2 are written, and while they are still in registers their R2C
 // along x (the next iteration's first pass, which needs no scale: c enters
 // at x_mid) goes to S.  The tile's spectrum rows are read completely before
 // any of them is rewritten, and no other workgroup touches them.
 template <int M, int E, int TP, int HE>
-__global__ void __launch_bounds__(TP *(M / E))
-    k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
+__device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S, float *__restrict__ u,
                  const float *__restrict__ cptr, double *__restrict__ partials,
                  const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
   constexpr int T = M / E;
@@ -914,7 +969,7 @@ __global__ void __launch_bounds__(TP *(M / E))
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -972,14 +1027,27 @@ __global__ void __launch_bounds__(TP *(M / E))
   if (threadIdx.x == 0) partials[tile] = sh[0];
 }
 
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
+                 const float *__restrict__ cptr, double *__restrict__ partials,
+                 const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  x_last_fwd_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, g);
+}
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    kb_x_last_fwd(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  x_last_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, v.g);
+}
+
 // ------------------------------------------------------------- y and z
 
 // H: the image is half the work domain along y (iy = N/2, a same-size
 // kernel), so the forward transform's nonzero inputs and the inverse's kept
 // outputs are exactly elements e < E/2 of every thread: no bounds tests.
 template <int N, int E, int PPW, int SIGN, bool H>
-__global__ void __launch_bounds__(PPW *(N / E))
-    k_y(float2 *__restrict__ S, const float2 *__restrict__ gtw, long long npencils,
+__device__ __forceinline__ void y_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ gtw, long long npencils,
         int Lin, int Lout) {
   constexpr int T = N / E;
   constexpr int NT = PPW * T;
@@ -990,7 +1058,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
   __shared__ float2 lds[PPW * NP];
   __shared__ float2 tw[N];
   const int pl = threadIdx.x / T, t = threadIdx.x % T;
-  const long long p = (long long)blockIdx.x * PPW + pl;
+  const long long p = (long long)blk * PPW + pl;
   const bool valid = p < npencils;
   float2 *base = S + p * N;
   // loads without branches (pencil and index clamped, value selected), so
@@ -1023,6 +1091,19 @@ __global__ void __launch_bounds__(PPW *(N / E))
   }
 }
 
+template <int N, int E, int PPW, int SIGN, bool H>
+__global__ void __launch_bounds__(PPW *(N / E))
+    k_y(float2 *__restrict__ S, const float2 *__restrict__ gtw, long long npencils,
+        int Lin, int Lout) {
+  y_body<N, E, PPW, SIGN, H>(blockIdx.x, S, gtw, npencils, Lin, Lout);
+}
+template <int N, int E, int PPW, int SIGN, bool H>
+__global__ void __launch_bounds__(PPW *(N / E))
+    kb_y(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  y_body<N, E, PPW, SIGN, H>((int)blockIdx.x - v.blk0, v.b.S, v.b.twY, (long long)v.g.iz * v.g.hx, SIGN < 0 ? v.g.iy : N, SIGN < 0 ? N : v.g.iy);
+}
+
 // A workgroup transforms TP adjacent ky columns of one kx (TP * 8 bytes
 // contiguous per z plane); TP lanes of a wavefront read one plane.  Each
 // column of S is multiplied by the matching OTF column, the z transform of
@@ -1033,8 +1114,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
 // are exactly elements e < E/2, and the kernel's are e < E/4 (planes
 // z < N/4) and e >= 3E/4 (planes z >= 3N/4): no bounds tests.
 template <int N, int E, int TP, bool H>
-__global__ void __launch_bounds__(TP *(N / E))
-    k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
+__device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ K,
         const float2 *__restrict__ gtw, Geom g, int flip) {
   static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
   constexpr int T = N / E;
@@ -1043,8 +1123,8 @@ __global__ void __launch_bounds__(TP *(N / E))
   __shared__ float2 tw[N];
   const int q = threadIdx.x % TP, t = threadIdx.x / TP;
   const int ntile = g.ny / TP;
-  const int kx = blockIdx.x / ntile;
-  const int y0 = (blockIdx.x % ntile) * TP;
+  const int kx = blk / ntile;
+  const int y0 = (blk % ntile) * TP;
   const long long col = (long long)kx * g.ny + y0 + q;
   // element (t + T*e) of the column: one 64-bit step of T planes per e
   const long long step = (long long)T * g.sz;
@@ -1126,6 +1206,19 @@ __global__ void __launch_bounds__(TP *(N / E))
       }
     }
   }
+}
+
+template <int N, int E, int TP, bool H>
+__global__ void __launch_bounds__(TP *(N / E))
+    k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
+        const float2 *__restrict__ gtw, Geom g, int flip) {
+  z_body<N, E, TP, H>(blockIdx.x, S, K, gtw, g, flip);
+}
+template <int N, int E, int TP, bool H>
+__global__ void __launch_bounds__(TP *(N / E))
+    kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
+  const VolTab &v = vt[vmap[blockIdx.x]];
+  z_body<N, E, TP, H>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.flip);
 }
 
 // The same-size z pass (iz = kz = N/2) over several column tiles per
@@ -1261,33 +1354,48 @@ bool z_supported(int N) {
   return N == 64 || N == 96 || N == 128 || N == 192 || N == 256 || N == 512;
 }
 
-struct Bufs {
-  float *u, *obs, *scal;  // scal: E, S, c
-  float2 *S, *K;          // spectrum; kernel after its x and y transforms
-  double *parts;
-  const float2 *twM, *twN, *twY, *twZ;
-};
 
 // Number of x tiles (= x_last partial sums) for a tile height.
 int x_tiles(const Geom &g, int tp) { return g.iz * ((g.iy + tp - 1) / tp); }
+
+// Record mode of the launch sites, for setting up a batched iteration: a
+// site reports the batched form of the kernel it would launch, with its
+// grid and block size, instead of launching it.
+struct LaunchRec {
+  const void *fn;
+  unsigned grid, block;
+};
+thread_local LaunchRec *g_rec = nullptr;
+
+template <class K>
+bool recorded(K *kb, unsigned grid, unsigned block) {
+  if (!g_rec) return false;
+  *g_rec = LaunchRec{reinterpret_cast<const void *>(kb), grid, block};
+  return true;
+}
 
 template <int M, int E, int TP, int HE>
 void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr int NT = TP * (M / E);
   const int nt = x_tiles(g, TP);
   const unsigned grid = (unsigned)(TP == 16 ? nt : (nt + 15) / 16 * 16);
-  if (kind == KX_FWD)
+  if (kind == KX_FWD) {
+    if (recorded(&kb_x_fwd<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.u, b.S,
                        b.twM, b.twN, g);
-  else if (kind == KX_MID)
+  } else if (kind == KX_MID) {
+    if (recorded(&kb_x_mid<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_mid<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.obs,
                        b.scal + 2, b.twM, b.twN, g);
-  else if (kind == KX_LAST_FWD)
+  } else if (kind == KX_LAST_FWD) {
+    if (recorded(&kb_x_last_fwd<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_last_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
-  else
+  } else {
+    if (recorded(&kb_x_last<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_last<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
+  }
 }
 
 // With h, a row that ends on an element boundary of every thread (ix =
@@ -1351,19 +1459,25 @@ void launch_y_p(bool fwd, const Geom &g, const Bufs &b, hipStream_t s) {
   const unsigned grid = (unsigned)((np + PPW - 1) / PPW);
   const dim3 blk(PPW * (N / E));
   if (2 * g.iy == N) {
-    if (fwd)
+    if (fwd) {
+      if (recorded(&kb_y<N, E, PPW, -1, true>, grid, blk.x)) return;
       hipLaunchKernelGGL((k_y<N, E, PPW, -1, true>), dim3(grid), blk, 0, s, b.S, b.twY, np,
                          g.iy, N);
-    else
+    } else {
+      if (recorded(&kb_y<N, E, PPW, +1, true>, grid, blk.x)) return;
       hipLaunchKernelGGL((k_y<N, E, PPW, +1, true>), dim3(grid), blk, 0, s, b.S, b.twY, np,
                          N, g.iy);
+    }
   } else {
-    if (fwd)
+    if (fwd) {
+      if (recorded(&kb_y<N, E, PPW, -1, false>, grid, blk.x)) return;
       hipLaunchKernelGGL((k_y<N, E, PPW, -1, false>), dim3(grid), blk, 0, s, b.S, b.twY, np,
                          g.iy, N);
-    else
+    } else {
+      if (recorded(&kb_y<N, E, PPW, +1, false>, grid, blk.x)) return;
       hipLaunchKernelGGL((k_y<N, E, PPW, +1, false>), dim3(grid), blk, 0, s, b.S, b.twY, np,
                          N, g.iy);
+    }
   }
 }
 
@@ -1401,15 +1515,18 @@ void launch_z_t(int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) 
   // planes (its out-of-range offset plus the largest element offset)
   const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
   const bool fits32 = 2 * extent < (1ull << 32);
-  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32 && !flip)
+  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32 && !flip && !g_rec)
     hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
                        dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
-  else if (2 * g.iz == N && 2 * g.kz == N)
+  else if (2 * g.iz == N && 2 * g.kz == N) {
+    if (recorded(&kb_z<N, E, TP, true>, grid, TP * (N / E))) return;
     hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
                        b.K, b.twZ, g, (int)flip);
-  else
+  } else {
+    if (recorded(&kb_z<N, E, TP, false>, grid, TP * (N / E))) return;
     hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
                        b.K, b.twZ, g, (int)flip);
+  }
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
@@ -1552,6 +1669,28 @@ class FastRL : public RLEngine {
       (void)hipEventDestroy(ev.first);
       (void)hipEventDestroy(ev.second);
     }
+  }
+
+  // The batched form of the launch `kind` makes (z_mod: of the iteration's
+  // second half when `second`), and this engine's entry of its table.
+  LaunchRec batched(Kind kind, bool second) const {
+    LaunchRec r{nullptr, 0, 0};
+    g_rec = &r;
+    switch (kind) {
+      case KX_FWD:
+      case KX_MID:
+      case KX_LAST:
+      case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, nullptr); break;
+      case KY_FWD: dispatch_y(tu_, true, g_, b_, nullptr); break;
+      case KY_INV: dispatch_y(tu_, false, g_, b_, nullptr); break;
+      case KZ_MOD: dispatch_z(tu_, g_, b_, nullptr, flip_ && second); break;
+      default: break;
+    }
+    g_rec = nullptr;
+    return r;
+  }
+  VolTab table_entry(bool second) const {
+    return VolTab{g_, b_, 0, (flip_ && second) ? 1 : 0, nparts_};
   }
 
   // The kernel half-step of alternating blind RL: the correction
@@ -1941,6 +2080,161 @@ class FastRL : public RLEngine {
   double ms_[KNUM] = {};
 };
 
+// The finish of several volumes, one block each: the sum of the volume's
+// partials in the order and rounding of the single-volume finish (256
+// threads, strided then a tree), S = (float)sum, c = E / S.
+__global__ void __launch_bounds__(256) kb_finish(const VolTab *__restrict__ vt) {
+  __shared__ double sh[256];
+  const VolTab &v = vt[blockIdx.x];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < v.nparts; i += blockDim.x) acc += v.b.parts[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float f = (float)sh[0];
+    v.b.scal[1] = f;
+    v.b.scal[2] = v.b.scal[0] / f;
+  }
+}
+
+// RL iterations of several fused engines together: for every pass, one
+// launch per kernel instantiation over all the volumes that use it (a
+// block-to-volume map and a table of the volumes' geometry and buffers),
+// instead of one launch per volume; every volume's finish in one launch.
+// Each volume runs the very kernels, arguments and order of its own
+// engine's chained iteration, so the results are those of its engine.
+class FastGroup : public RLGroup {
+ public:
+  explicit FastGroup(std::vector<FastRL *> e) : e_(std::move(e)) {}
+  ~FastGroup() override {
+    for (auto &ge : graph_)
+      if (ge) (void)hipGraphExecDestroy(ge);
+  }
+
+  ClarityResult_t build() {
+    const Kind kinds[] = {KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KX_LAST_FWD};
+    for (Kind k : kinds)
+      for (int half = 0; half < 2; half++) {
+        if (half == 1 && k != KZ_MOD) continue;
+        // volumes by (batched kernel, block size), in volume order
+        std::vector<std::pair<LaunchRec, std::vector<int>>> groups;
+        for (int v = 0; v < (int)e_.size(); v++) {
+          const LaunchRec r = e_[v]->batched(k, half == 1);
+          if (!r.fn || r.grid == 0) return CLARITY_INVALID_OPERATION;
+          auto it = groups.begin();
+          for (; it != groups.end(); ++it)
+            if (it->first.fn == r.fn && it->first.block == r.block) break;
+          if (it == groups.end()) {
+            groups.push_back({r, {}});
+            it = groups.end() - 1;
+          }
+          it->second.push_back(v);
+        }
+        for (auto &gr : groups) {
+          auto L = std::unique_ptr<Launch>(new Launch);
+          L->fn = gr.first.fn;
+          L->block = gr.first.block;
+          std::vector<VolTab> tab;
+          std::vector<unsigned> map;
+          for (int v : gr.second) {
+            VolTab t = e_[v]->table_entry(half == 1);
+            t.blk0 = (int)map.size();
+            const unsigned grid = e_[v]->batched(k, half == 1).grid;
+            map.insert(map.end(), grid, (unsigned)tab.size());
+            tab.push_back(t);
+          }
+          L->grid = (unsigned)map.size();
+          CL_TRY(L->tab.alloc(sizeof(VolTab) * tab.size()));
+          CL_TRY(L->map.alloc(sizeof(unsigned) * map.size()));
+          CL_HIP(hipMemcpy(L->tab.p, tab.data(), sizeof(VolTab) * tab.size(),
+                           hipMemcpyHostToDevice));
+          CL_HIP(hipMemcpy(L->map.p, map.data(), sizeof(unsigned) * map.size(),
+                           hipMemcpyHostToDevice));
+          launches_[k][half].push_back(std::move(L));
+        }
+      }
+    std::vector<VolTab> fin;
+    for (FastRL *e : e_) fin.push_back(e->table_entry(false));
+    CL_TRY(fin_.alloc(sizeof(VolTab) * fin.size()));
+    CL_HIP(hipMemcpy(fin_.p, fin.data(), sizeof(VolTab) * fin.size(), hipMemcpyHostToDevice));
+    return CLARITY_SUCCESS;
+  }
+
+  // n chained iterations of every volume, as FastRL::iterate: x_fwd, then
+  // n bodies, the last ending in x_last (replayed from two captured graphs
+  // unless CLARITY_NO_GRAPH is set).
+  ClarityResult_t iterate(int n, hipStream_t s) override {
+    if (n <= 0) return CLARITY_SUCCESS;
+    CL_TRY(pass(KX_FWD, 0, s));
+    if (std::getenv("CLARITY_NO_GRAPH")) {
+      for (int it = 0; it < n; it++) CL_TRY(body(it + 1 < n, s));
+      return CLARITY_SUCCESS;
+    }
+    for (int k = 0; k < 2; k++)
+      if (!graph_[k]) CL_TRY(capture(k == 0, &graph_[k], s));
+    for (int it = 0; it + 1 < n; it++) CL_HIP(hipGraphLaunch(graph_[0], s));
+    CL_HIP(hipGraphLaunch(graph_[1], s));
+    return CLARITY_SUCCESS;
+  }
+
+ private:
+  struct Launch {
+    const void *fn = nullptr;
+    unsigned grid = 0, block = 0;
+    DevBuf tab, map;
+  };
+
+  ClarityResult_t pass(Kind k, int half, hipStream_t s) {
+    for (auto &L : launches_[k][half]) {
+      const VolTab *t = static_cast<const VolTab *>(L->tab.p);
+      const unsigned *m = static_cast<const unsigned *>(L->map.p);
+      void *args[] = {&t, &m};
+      CL_HIP(hipLaunchKernel(L->fn, dim3(L->grid), dim3(L->block), args, 0, s));
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  ClarityResult_t body(bool chained, hipStream_t s) {
+    for (int half = 0; half < 2; half++) {
+      CL_TRY(pass(KY_FWD, 0, s));
+      CL_TRY(pass(KZ_MOD, half, s));
+      CL_TRY(pass(KY_INV, 0, s));
+      CL_TRY(pass(half == 0 ? KX_MID : chained ? KX_LAST_FWD : KX_LAST, 0, s));
+    }
+    const VolTab *t = static_cast<const VolTab *>(fin_.p);
+    hipLaunchKernelGGL(kb_finish, dim3((unsigned)e_.size()), dim3(256), 0, s, t);
+    return launch_check();
+  }
+
+  ClarityResult_t capture(bool chained, hipGraphExec_t *out, hipStream_t s) {
+    hipGraph_t graph = nullptr;
+    CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const ClarityResult_t r = body(chained, s);
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (r != CLARITY_SUCCESS) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return r;
+    }
+    CL_HIP(e);
+    const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      *out = nullptr;
+      return from_hip(ei);
+    }
+    return CLARITY_SUCCESS;
+  }
+
+  std::vector<FastRL *> e_;
+  std::vector<std::unique_ptr<Launch>> launches_[KNUM][2];
+  DevBuf fin_;
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+};
+
 }  // namespace
 
 // Work domain of the fused engine for an RL plan.
@@ -2038,6 +2332,19 @@ std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &ref) {
   auto *e = new FastRL(geom);
   e->make_kernel_step();
   return std::unique_ptr<RLEngine>(e);
+}
+
+std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines) {
+  std::vector<FastRL *> f;
+  for (RLEngine *e : engines) {
+    auto *x = dynamic_cast<FastRL *>(e);
+    if (!x) return nullptr;
+    f.push_back(x);
+  }
+  if (f.empty() || f.size() > 65535) return nullptr;
+  auto g = std::unique_ptr<FastGroup>(new FastGroup(std::move(f)));
+  if (g->build() != CLARITY_SUCCESS) return nullptr;
+  return g;
 }
 
 }  // namespace clarity

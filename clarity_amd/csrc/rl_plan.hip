@@ -224,15 +224,30 @@ ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
     if (rc == CLARITY_SUCCESS) rc = p->eng->reset(nullptr, s);
     p->have_inputs = true;
   }
-  // interleave in rounds of a few iterations of every volume, so all streams
-  // stay busy until the end; within a round a volume's iterations chain
-  // (the fused engine's x_last runs into the next x_fwd) and replay its
-  // captured graphs
-  constexpr int kRound = 10;
-  for (int done = 0; done < iterations && rc == CLARITY_SUCCESS; done += kRound) {
-    const int n = iterations - done < kRound ? iterations - done : kRound;
-    for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++)
-      rc = vols[v].plan->eng->iterate(n, streams[v % nstreams]);
+  // All fused: one launch per pass and kernel instantiation over every
+  // volume (make_rl_group), on the first stream once the set-up is done.
+  // Otherwise (or with CLARITY_BATCH_STREAMS set) each volume on its own
+  // plan, interleaved in rounds of a few iterations over the streams, so
+  // all stay busy until the end; within a round a volume's iterations
+  // chain and replay its captured graphs.
+  std::unique_ptr<RLGroup> group;
+  if (rc == CLARITY_SUCCESS && count > 1 && !std::getenv("CLARITY_BATCH_STREAMS")) {
+    std::vector<RLEngine *> engines;
+    for (int v = 0; v < count; v++) engines.push_back(vols[v].plan->eng.get());
+    group = make_rl_group(engines);
+  }
+  if (group) {
+    for (int i = 0; i < nstreams && rc == CLARITY_SUCCESS; i++)
+      rc = from_hip(hipStreamSynchronize(streams[i]));
+    if (rc == CLARITY_SUCCESS) rc = group->iterate(iterations, streams[0]);
+    if (rc == CLARITY_SUCCESS) rc = from_hip(hipStreamSynchronize(streams[0]));
+  } else {
+    constexpr int kRound = 10;
+    for (int done = 0; done < iterations && rc == CLARITY_SUCCESS; done += kRound) {
+      const int n = iterations - done < kRound ? iterations - done : kRound;
+      for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++)
+        rc = vols[v].plan->eng->iterate(n, streams[v % nstreams]);
+    }
   }
   for (int v = 0; v < count && rc == CLARITY_SUCCESS; v++) {
     hipStream_t s = streams[v % nstreams];
@@ -247,7 +262,8 @@ ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
     ClarityResult_t r2 = from_hip(hipStreamSynchronize(s));
     if (rc == CLARITY_SUCCESS) rc = r2;
   }
-  vols.clear();  // frees device buffers while the streams still exist
+  group.reset();  // its graphs refer to the volumes' buffers
+  vols.clear();   // frees device buffers while the streams still exist
   for (hipStream_t s : streams)
     if (s) (void)hipStreamDestroy(s);
   return rc;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 #include "work.hpp"
@@ -59,5 +60,15 @@ std::unique_ptr<RLEngine> make_fast_rl(const Work &geom);
 // (estimate = the kernel, blurring input = the image estimate; same even
 // image and kernel dimensions, reference work domain); nullptr otherwise.
 std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &geom);
+
+// RL iterations of several engines together (for each pass, one launch per
+// kernel instantiation over all the volumes that use it).
+class RLGroup {
+ public:
+  virtual ~RLGroup() = default;
+  virtual ClarityResult_t iterate(int n, hipStream_t s) = 0;
+};
+// nullptr unless every engine is a fused one (allocated, inputs set).
+std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines);
 
 }  // namespace clarity

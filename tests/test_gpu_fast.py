@@ -294,22 +294,35 @@ def test_z_pipelined_matches_one_tile(clarity_gpu, ishape, tune):
     assert rel_l2(outs[tune], oracle.ml(obs, psf, 5)) <= 1e-4
 
 
-def test_batch_uneven_volumes(clarity_gpu):
-    """Batched RL over uneven volumes equals per-volume RL (each volume's
-    result is the single-volume call's, to float rounding of the engine)."""
+@pytest.mark.parametrize("path", ["grouped", "streams"])
+def test_batch_uneven_volumes(clarity_gpu, path):
+    """Batched RL over uneven volumes equals per-volume RL.  Grouped (the
+    default when every volume is on the fused engine): one launch per pass
+    and kernel instantiation over all volumes, each volume running its own
+    engine's kernels, arguments and order, so bit for bit the single-volume
+    call.  Streams (CLARITY_BATCH_STREAMS): each volume on its own plan."""
     api = clarity_gpu
     rng = np.random.default_rng(44)
-    shapes = [(32, 64, 64), (16, 48, 40), (24, 96, 64), (32, 64, 64), (8, 20, 30)]
+    shapes = [(32, 64, 64), (16, 48, 40), (24, 96, 64), (32, 64, 64), (8, 20, 30),
+              (48, 128, 96), (8, 20, 30)]
     imgs, ks = [], []
     for i, s in enumerate(shapes):
         obj = synth.beads_and_filaments(s, 30, 2, rng)
         k = synth.gaussian_kernel(s, rng.uniform(1.2, 2.5), rng.uniform(2.5, 5.0))
         imgs.append(synth.poisson(synth.np_convolve(obj, k), rng))
         ks.append(k)
-    outs = api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 6)
+    if path == "streams":
+        os.environ["CLARITY_BATCH_STREAMS"] = "1"
+    try:
+        outs = api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 6)
+    finally:
+        os.environ.pop("CLARITY_BATCH_STREAMS", None)
     for a, k, o in zip(imgs, ks, outs):
         single = api.MaximumLikelihoodDeconvolve(a, k, 6)
-        assert rel_l2(o, single) <= 1e-6
+        if path == "grouped":
+            assert np.array_equal(o, single)
+        else:
+            assert rel_l2(o, single) <= 1e-6
         assert rel_l2(o, oracle.ml(a, k, 6)) <= 1e-4
 
 
