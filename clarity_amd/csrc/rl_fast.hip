@@ -141,6 +141,74 @@ __device__ __forceinline__ void dft6(float2 (&u)[6]) {
   u[1] = b0; u[3] = b1; u[5] = b2;
 }
 
+// DFT4 with zero inputs Z (bit n: u[n] = 0) and outputs k < KN only.
+template <int Z, int KN>
+__device__ __forceinline__ void dft4_p(float2 (&u)[4], int sign) {
+  constexpr bool z1 = Z & 2, z2 = Z & 4, z3 = Z & 8;
+  const float2 t0 = z2 ? u[0] : cadd(u[0], u[2]);
+  const float2 t1 = z2 ? u[0] : csub(u[0], u[2]);
+  const float2 t2 = z3 ? u[1] : z1 ? u[3] : cadd(u[1], u[3]);
+  const float2 d = z3 ? u[1] : z1 ? make_float2(-u[3].x, -u[3].y) : csub(u[1], u[3]);
+  const float2 t3 = cmul_si(d, sign);
+  u[0] = cadd(t0, t2);
+  u[1] = cadd(t1, t3);
+  if constexpr (KN > 2) u[2] = csub(t0, t2);
+  if constexpr (KN > 3) u[3] = csub(t1, t3);
+}
+
+// a * exp(SIGN * 2 pi i j / 12) for a compile-time j
+template <int SIGN, int J>
+__device__ __forceinline__ float2 rot12(float2 a) {
+  constexpr int j = J % 12;
+  constexpr float h = 0.86602540378443864676f;  // sin(pi/3)
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (j == 3) {
+    return cmul_si(a, SIGN);
+  } else if constexpr (j == 6) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (j == 9) {
+    return cmul_si(a, -SIGN);
+  } else {
+    // cos and sin of 2 pi j / 12 for j = 1, 2, 4, 5, 7, 8, 10, 11
+    constexpr float c = (j == 1 || j == 11) ? h : (j == 2 || j == 10) ? 0.5f
+                        : (j == 4 || j == 8) ? -0.5f : -h;
+    constexpr float sn = (j == 1 || j == 5) ? 0.5f : (j == 2 || j == 4) ? h
+                         : (j == 7 || j == 11) ? -0.5f : -h;
+    constexpr float sg = SIGN * sn;
+    return make_float2(a.x * c - a.y * sg, a.x * sg + a.y * c);
+  }
+}
+
+// 12-point DFT: n = 3*n1 + n2, k = k1 + 4*k2; three DFT4 over n1, twiddle
+// w12^(n2*k1), four DFT3 over n2.  Z (bit n1: inputs with that n1 are
+// zero) prunes the DFT4s as dft4_p.
+template <int SIGN, int Z = 0>
+__device__ __forceinline__ void dft12(float2 (&u)[12]) {
+  float2 a[3][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; n2++) {
+    float2 t[4] = {u[n2], u[3 + n2], u[6 + n2], u[9 + n2]};
+    dft4_p<Z, 4>(t, SIGN);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) a[n2][k1] = t[k1];
+  }
+  a[1][1] = rot12<SIGN, 1>(a[1][1]);
+  a[1][2] = rot12<SIGN, 2>(a[1][2]);
+  a[1][3] = rot12<SIGN, 3>(a[1][3]);
+  a[2][1] = rot12<SIGN, 2>(a[2][1]);
+  a[2][2] = rot12<SIGN, 4>(a[2][2]);
+  a[2][3] = rot12<SIGN, 6>(a[2][3]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    float2 b0 = a[0][k1], b1 = a[1][k1], b2 = a[2][k1];
+    dft3<SIGN>(b0, b1, b2);
+    u[k1] = b0;
+    u[k1 + 4] = b1;
+    u[k1 + 8] = b2;
+  }
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft(float2 (&u)[R]) {
   if constexpr (R == 1) {
@@ -156,6 +224,8 @@ __device__ __forceinline__ void dft(float2 (&u)[R]) {
     dft3<SIGN>(u[0], u[1], u[2]);
   } else if constexpr (R == 6) {
     dft6<SIGN>(u);
+  } else if constexpr (R == 12) {
+    dft12<SIGN>(u);
   } else {
     static_assert(R == 16, "unsupported radix");
   }
@@ -171,20 +241,6 @@ __device__ __forceinline__ void dft(float2 (&u)[R]) {
 // in IEEE arithmetic, -0 + 0 being +0).
 constexpr int kZeroUpper = 1, kZeroMiddle = 2, kKeepLower = 1;
 
-// DFT4 with zero inputs Z (bit n: u[n] = 0) and outputs k < KN only.
-template <int Z, int KN>
-__device__ __forceinline__ void dft4_p(float2 (&u)[4], int sign) {
-  constexpr bool z1 = Z & 2, z2 = Z & 4, z3 = Z & 8;
-  const float2 t0 = z2 ? u[0] : cadd(u[0], u[2]);
-  const float2 t1 = z2 ? u[0] : csub(u[0], u[2]);
-  const float2 t2 = z3 ? u[1] : z1 ? u[3] : cadd(u[1], u[3]);
-  const float2 d = z3 ? u[1] : z1 ? make_float2(-u[3].x, -u[3].y) : csub(u[1], u[3]);
-  const float2 t3 = cmul_si(d, sign);
-  u[0] = cadd(t0, t2);
-  u[1] = cadd(t1, t3);
-  if constexpr (KN > 2) u[2] = csub(t0, t2);
-  if constexpr (KN > 3) u[3] = csub(t1, t3);
-}
 
 template <int SIGN, int ZP, int KP>
 __device__ __forceinline__ void dft16_p(float2 (&u)[16]) {
@@ -254,6 +310,9 @@ __device__ __forceinline__ void dft_p(float2 (&u)[R]) {
     dft16_p<SIGN, ZP, KP>(u);
   } else if constexpr (R == 8) {
     dft8_p<SIGN, ZP, KP>(u);
+  } else if constexpr (R == 12) {
+    // n = 3*n1 + n2: upper half zero is n1 >= 2, middle half zero n1 in {1, 2}
+    dft12<SIGN, ZP == kZeroUpper ? 0xC : ZP == kZeroMiddle ? 0x6 : 0>(u);
   } else if constexpr (R == 2 && ZP == 0) {
     u[0] = cadd(u[0], u[1]);
   } else {
@@ -429,15 +488,16 @@ __device__ __forceinline__ void fft_stages(float2 (&v)[E], int t, const L &lds,
 
 // Radix plan for an N-point transform with E elements per thread: powers of
 // two use radix-E stages then the remainder; 3 * 2^k sizes use E = 12 with a
-// radix 3 or 6 stage first.
+// radix-12 stage first (one LDS exchange fewer than radix 3 then 4s), but
+// 384, where radix 12 would leave 32 = 4 * 4 * 2 and no stage is saved.
 template <int N, int E>
 struct Plan {
   using Rad = DefaultRadices<N, E>;
 };
-template <> struct Plan<96, 12> { using Rad = Radices<3, 4, 4, 2>; };
-template <> struct Plan<192, 12> { using Rad = Radices<3, 4, 4, 4>; };
+template <> struct Plan<96, 12> { using Rad = Radices<12, 4, 2>; };
+template <> struct Plan<192, 12> { using Rad = Radices<12, 4, 4>; };
 template <> struct Plan<384, 12> { using Rad = Radices<6, 4, 4, 4>; };
-template <> struct Plan<768, 12> { using Rad = Radices<3, 4, 4, 4, 4>; };
+template <> struct Plan<768, 12> { using Rad = Radices<12, 4, 4, 4>; };
 
 struct Geom {
   int ix, iy, iz;  // image
@@ -1605,7 +1665,10 @@ Tune pick_tune(const Geom &g) {
     }
   }
   t.yp = 4;
-  t.ze = (g.nz == 128) ? 8 : 16;
+  // 8 elements per thread at nz 128 only for the general z kernel; the
+  // same-size one runs faster with 16 (measured at 64x256x256: 25 against
+  // 32 us)
+  t.ze = (g.nz == 128 && !(2 * g.iz == g.nz && 2 * g.kz == g.nz)) ? 8 : 16;
   t.zt = (g.nz >= 256) ? 32 : 16;  // measured: 256-byte plane segments at nz 256
   // at nz 512 the general z kernel (both columns' bounds tested) spills
   // with 32 columns (1024 threads, so at most 128 VGPRs); 16 do not
