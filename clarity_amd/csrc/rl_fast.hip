@@ -938,10 +938,27 @@ __global__ void __launch_bounds__(TP *(M / E))
   x_mid_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.obs, v.b.scal + 2, v.b.twM, v.b.twN, v.g);
 }
 
-template <int M, int E, int TP, int HE>
+// Jansen-van Cittert update (reference src/JansenVanCittertDeconvolve.cxx:
+// 106-112) with o the blurred estimate: gamma = 1 - (o - A)^2 / A^2, u' =
+// max(0, u + gamma (in - o)), without contraction, as the reference's float
+// code and the generic driver.
+__device__ __forceinline__ float jvc_update(float u, float in, float o, float A,
+                                            float invASq) {
+#pragma clang fp contract(off)
+  const float diff = o - A;
+  const float gamma = 1.0f - ((diff * diff) * invASq);
+  const float val = u + (gamma * (in - o));
+  return val < 0.0f ? 0.0f : val;
+}
+
+// MODE 0: the RL update u' = (c u) s2 and its sum; MODE 1: the JVC update
+// of u with the blurred estimate s2 and the observation row (cptr: A and
+// 1/A^2), no sum.
+template <int M, int E, int TP, int HE, int MODE = 0>
 __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
-             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+             const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN,
+             const float *__restrict__ obs, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
   constexpr int PS = kPadShift<E>;
@@ -964,6 +981,8 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
   tn.load(gtwN);
   float2 uo[E];
   load_row_pairs<E, T, HE>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  float2 ob[MODE == 1 ? E : 1];
+  if constexpr (MODE == 1) load_row_pairs<E, T, HE>(ob, load_row(obs, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -973,6 +992,20 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  if constexpr (MODE == 1) {
+    const float A = cptr[0], invASq = cptr[1];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int n0 = 2 * (t + T * e);
+      if (valid && in_image<HE>(e, n0, g.ix)) {
+        const bool second = second_in_image<HE>(e, n0, g.ix);
+        const float u0 = jvc_update(uo[e].x, ob[e].x, v[e].x, A, invASq);
+        const float u1 = second ? jvc_update(uo[e].y, ob[e].y, v[e].y, A, invASq) : 0.f;
+        store_row_pair(urow, n0, g.ix, u0, u1);
+      }
+    }
+    return;
+  }
   // u' = (c * u) * s2: the reference's g' = g * s2 with g = c * u stored
   // rounded (src/MaximumLikelihoodDeconvolve.cxx:69-78)
   const float c = cptr[0];
@@ -1005,13 +1038,13 @@ __global__ void __launch_bounds__(TP *(M / E))
     k_x_last(const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
              const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
-  x_last_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, g);
+  x_last_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, nullptr, g);
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     kb_x_last(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
-  x_last_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, v.g);
+  x_last_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, nullptr, v.g);
 }
 
 // x_last of one iteration fused with x_fwd of the next: the updated rows
@@ -1019,10 +1052,11 @@ __global__ void __launch_bounds__(TP *(M / E))
 // along x (the next iteration's first pass, which needs no scale: c enters
 // at x_mid) goes to S.  The tile's spectrum rows are read completely before
 // any of them is rewritten, and no other workgroup touches them.
-template <int M, int E, int TP, int HE>
+template <int M, int E, int TP, int HE, int MODE = 0>
 __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S, float *__restrict__ u,
                  const float *__restrict__ cptr, double *__restrict__ partials,
-                 const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+                 const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN,
+             const float *__restrict__ obs, Geom g) {
   constexpr int T = M / E;
   constexpr int NT = TP * T;
   constexpr int PS = kPadShift<E>;
@@ -1045,6 +1079,8 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   tn.load(gtwN);
   float2 uo[E];
   load_row_pairs<E, T, HE>(uo, load_row(u, g, z, y), valid, t, g.ix);
+  float2 ob[MODE == 1 ? E : 1];
+  if constexpr (MODE == 1) load_row_pairs<E, T, HE>(ob, load_row(obs, g, z, y), valid, t, g.ix);
   sp.template store<PS>(sm.tile);
   tm.store(sm.twM);
   tn.store(sm.twN);
@@ -1054,7 +1090,7 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
-  // the same update and sum as k_x_last
+  // the same update (and for RL the same sum) as x_last
   const float c = cptr[0];
   double acc = 0.0;
 #pragma unroll
@@ -1063,8 +1099,13 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
     float u0 = 0.f, u1 = 0.f;
     if (valid && in_image<HE>(e, n0, g.ix)) {
       const bool second = second_in_image<HE>(e, n0, g.ix);
-      u0 = (c * uo[e].x) * v[e].x;
-      if (second) u1 = (c * uo[e].y) * v[e].y;
+      if constexpr (MODE == 1) {
+        u0 = jvc_update(uo[e].x, ob[e].x, v[e].x, c, cptr[1]);
+        if (second) u1 = jvc_update(uo[e].y, ob[e].y, v[e].y, c, cptr[1]);
+      } else {
+        u0 = (c * uo[e].x) * v[e].x;
+        if (second) u1 = (c * uo[e].y) * v[e].y;
+      }
       store_row_pair(urow, n0, g.ix, u0, u1);
       acc += (double)u0;
       if (second) acc += (double)u1;
@@ -1076,6 +1117,7 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
+  if constexpr (MODE == 1) return;
   double *sh = reinterpret_cast<double *>(sm.tile);
   __syncthreads();
   sh[threadIdx.x] = acc;
@@ -1092,13 +1134,30 @@ __global__ void __launch_bounds__(TP *(M / E))
     k_x_last_fwd(float2 *__restrict__ S, float *__restrict__ u,
                  const float *__restrict__ cptr, double *__restrict__ partials,
                  const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
-  x_last_fwd_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, g);
+  x_last_fwd_body<M, E, TP, HE>(blockIdx.x, S, u, cptr, partials, gtwM, gtwN, nullptr, g);
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
     kb_x_last_fwd(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
-  x_last_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, v.g);
+  x_last_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, nullptr, v.g);
+}
+
+// The last x pass of a Jansen-van Cittert iteration (C2R, update) and the
+// same fused with the next iteration's R2C.
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_jvc(const float2 *__restrict__ S, float *__restrict__ u, const float *__restrict__ ap,
+            const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN,
+            const float *__restrict__ obs, Geom g) {
+  x_last_body<M, E, TP, HE, 1>(blockIdx.x, S, u, ap, nullptr, gtwM, gtwN, obs, g);
+}
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_jvc_fwd(float2 *__restrict__ S, float *__restrict__ u, const float *__restrict__ ap,
+                const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN,
+                const float *__restrict__ obs, Geom g) {
+  x_last_fwd_body<M, E, TP, HE, 1>(blockIdx.x, S, u, ap, nullptr, gtwM, gtwN, obs, g);
 }
 
 // ------------------------------------------------------------- y and z
@@ -1379,7 +1438,10 @@ __global__ void __launch_bounds__(TP *(N / E)) __attribute__((amdgpu_waves_per_e
 
 // ------------------------------------------------------------ dispatch
 
-enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KX_LAST_FWD, KNUM };
+enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KX_LAST_FWD, KNUM,
+            // the Jansen-van Cittert engine's last x passes (timed as x_last
+            // and x_last_fwd)
+            KX_JVC, KX_JVC_FWD };
 const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod",  "y_inv",
                                   "x_mid", "x_last", "finish", "x_last_fwd"};
 
@@ -1451,10 +1513,17 @@ void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
     if (recorded(&kb_x_last_fwd<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_last_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
-  } else {
+  } else if (kind == KX_LAST) {
     if (recorded(&kb_x_last<M, E, TP, HE>, grid, NT)) return;
     hipLaunchKernelGGL((k_x_last<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal + 2, b.parts, b.twM, b.twN, g);
+  } else if (kind == KX_JVC) {
+    // A and 1/A^2 in the scalars' first two slots
+    hipLaunchKernelGGL((k_x_jvc<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u, b.scal,
+                       b.twM, b.twN, b.obs, g);
+  } else if (kind == KX_JVC_FWD) {
+    hipLaunchKernelGGL((k_x_jvc_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
+                       b.scal, b.twM, b.twN, b.obs, g);
   }
 }
 
@@ -1498,7 +1567,8 @@ void launch_x(int e, int tp, bool h, Kind kind, const Geom &g, const Bufs &b,
 }
 
 void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
-  const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1 : (kind == KX_LAST) ? 2 : 3;
+  const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1
+                : (kind == KX_LAST || kind == KX_JVC) ? 2 : 3;
   const int jt = j < 3 ? j : 2;  // x_last_fwd takes x_last's launch shape
   const int e = tu.xe[jt], tp = tu.xt[jt];
   const bool h = (tu.xh >> j) & 1;  // the launcher checks that the rows allow it
@@ -1761,6 +1831,16 @@ class FastRL : public RLEngine {
   // renormalised to sum 1 (see make_fast_rl_kernel_step).
   void make_kernel_step() { flip_ = unit_ = true; }
 
+  // Jansen-van Cittert instead of RL (make_fast_jvc): each iteration one
+  // convolution and u' = max(0, u + gamma (obs - H (*) u)); A and 1/A^2 are
+  // set after set_inputs (the first two scalar slots, unused by JVC).
+  void make_jvc() { jvc_ = true; }
+  bool is_jvc() const { return jvc_; }
+  ClarityResult_t set_jvc_constants(float A, float invASq, hipStream_t s) override {
+    CL_TRY(k_fill(scal_.f(), 1, A, s));
+    return k_fill(scal_.f() + 1, 1, invASq, s);
+  }
+
   static bool supports(const Work &w) {
     return w.wd.x % 2 == 0 && x_supported(w.wd.x / 2) && y_supported(w.wd.y) &&
            z_supported(w.wd.z) && w.wd.y % kTPz == 0;
@@ -1887,6 +1967,13 @@ class FastRL : public RLEngine {
   // One iteration after its x_fwd: nine launches (the last x pass fused
   // with the next iteration's x_fwd when `chained`).
   ClarityResult_t enqueue_body(bool chained, hipStream_t s) {
+    if (jvc_) {
+      // one convolution, then the JVC update in the last x pass
+      CL_TRY(run(KY_FWD, s));
+      CL_TRY(run(KZ_MOD, s));
+      CL_TRY(run(KY_INV, s));
+      return run(chained ? KX_JVC_FWD : KX_JVC, s);
+    }
     for (int half = 0; half < 2; half++) {
       CL_TRY(run(KY_FWD, s));
       flip_now_ = flip_ && half == 1;
@@ -2108,7 +2195,9 @@ class FastRL : public RLEngine {
       case KX_FWD:
       case KX_MID:
       case KX_LAST:
-      case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
+      case KX_LAST_FWD:
+      case KX_JVC:
+      case KX_JVC_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
       case KZ_MOD: dispatch_z(tu_, g_, b_, s, flip_now_); break;
@@ -2120,7 +2209,8 @@ class FastRL : public RLEngine {
     CL_TRY(launch_check());
     if (timing_) {
       CL_HIP(hipEventRecord(ev_pool_[slot].second, s));
-      pending_.push_back({kind, slot});
+      pending_.push_back({kind == KX_JVC ? KX_LAST : kind == KX_JVC_FWD ? KX_LAST_FWD : kind,
+                          slot});
     }
     return CLARITY_SUCCESS;
   }
@@ -2135,6 +2225,7 @@ class FastRL : public RLEngine {
   bool flip_ = false;      // kernel step: the correction's spectrum sign
   bool unit_ = false;      // kernel step: renormalise to sum 1
   bool flip_now_ = false;  // the z_mod being enqueued takes the sign
+  bool jvc_ = false;       // Jansen-van Cittert iterations
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_next_ = 0;
@@ -2397,11 +2488,32 @@ std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &ref) {
   return std::unique_ptr<RLEngine>(e);
 }
 
+// Jansen-van Cittert on the fused passes: its estimate spreads over the
+// whole work domain (u + gamma (obs - H (*) u) is not zero outside the
+// observation), so the engine's "image" is the reference domain itself
+// (every row, pencil and plane transformed) and there is no smaller
+// alias-free domain; the kernel planes and K are as for RL.  nullptr when
+// the domain is not one of the engine's sizes.
+std::unique_ptr<RLEngine> make_fast_jvc(const Work &ref) {
+  Work geom;
+  geom.set_geometry(ref);
+  geom.id = ref.wd;
+  geom.wd = ref.wd;
+  geom.IV = geom.wd.voxels();
+  geom.V = geom.wd.voxels();
+  geom.C = (long long)geom.wd.z * geom.wd.y * (geom.wd.x / 2 + 1);
+  geom.scale = inv_volume(geom.wd.x, geom.wd.y, geom.wd.z);
+  if (!FastRL::supports(geom)) return nullptr;
+  auto *e = new FastRL(geom);
+  e->make_jvc();
+  return std::unique_ptr<RLEngine>(e);
+}
+
 std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines) {
   std::vector<FastRL *> f;
   for (RLEngine *e : engines) {
     auto *x = dynamic_cast<FastRL *>(e);
-    if (!x) return nullptr;
+    if (!x || x->is_jvc()) return nullptr;
     f.push_back(x);
   }
   if (f.empty() || f.size() > 65535) return nullptr;

@@ -40,6 +40,12 @@ class RLEngine {
     (void)count; (void)ms; (void)cap; (void)reset;
     return 0;
   }
+  // Jansen-van Cittert engines (make_fast_jvc): the update's A and 1/A^2,
+  // after set_inputs.
+  virtual ClarityResult_t set_jvc_constants(float A, float invASq, hipStream_t s) {
+    (void)A; (void)invASq; (void)s;
+    return CLARITY_INVALID_OPERATION;
+  }
   // Picks launch shapes by timing candidates on the current inputs (the
   // estimate is saved and restored); writes a description of the choice to
   // `desc` (cap bytes).  Engines without tunable shapes do nothing.
@@ -60,6 +66,11 @@ std::unique_ptr<RLEngine> make_fast_rl(const Work &geom);
 // (estimate = the kernel, blurring input = the image estimate; same even
 // image and kernel dimensions, reference work domain); nullptr otherwise.
 std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &geom);
+
+// Jansen-van Cittert iterations on the fused passes over the reference work
+// domain (its "image" is the padded observation, its estimate the whole
+// domain); nullptr when the domain is not one of the fused engine's sizes.
+std::unique_ptr<RLEngine> make_fast_jvc(const Work &geom);
 
 // RL iterations of several engines together (for each pass, one launch per
 // kernel instantiation over all the volumes that use it).

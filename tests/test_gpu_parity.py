@@ -238,12 +238,30 @@ def test_ml_mixed_radix_reference_geometry(clarity_gpu):
     assert rel_l2(got, ref) <= 1e-4
 
 
-def test_jvc_vs_oracle(clarity_gpu, small):
+@pytest.mark.parametrize("engine", ["fused", "generic"])
+@pytest.mark.parametrize("name", ["beads64", "96x64x48"])
+def test_jvc_vs_oracle(clarity_gpu, small, name, engine):
+    """JVC on the fused passes over the reference domain (domains 128 x 128
+    x 64 and, radix 3, 192 x 128 x 96) and on the generic driver
+    (CLARITY_FORCE_GENERIC), against the oracle; 0 iterations run one, as
+    the reference."""
     api = clarity_gpu
+    if name == "beads64":
+        obs, psf = small.obs, small.psf
+    else:
+        case = synth.make_workload("beads-64x64x32", seed=5, shape=(48, 64, 96))
+        obs, psf = case.obs, case.psf
     for iters in (0, 1, 5):
-        got = api.JansenVanCittertDeconvolve(small.obs, small.psf, iters)
-        ref = oracle.jvc(small.obs, small.psf, iters)
-        assert rel_l2(got, ref) <= 1e-4, iters
+        if engine == "generic":
+            os.environ["CLARITY_FORCE_GENERIC"] = "1"
+        try:
+            got = api.JansenVanCittertDeconvolve(obs, psf, iters)
+        finally:
+            os.environ.pop("CLARITY_FORCE_GENERIC", None)
+        ref = oracle.jvc(obs, psf, iters)
+        err = rel_l2(got, ref)
+        print(f"[jvc] {name} {engine} {iters} it: rel L2 {err:.2e}", flush=True)
+        assert err <= 1e-4, iters
 
 
 def test_blind_as_written_vs_oracle(clarity_gpu, small):
