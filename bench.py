@@ -191,6 +191,10 @@ def suite(api, threads):
     emit("rl 100 iterations 256x256x64 (C API, host buffers)", g, 100 * loop,
          dense_bytes(shape, 4 * 100, 18 * 100), "cpu: 100 x per-iteration loop time of 2")
     k0 = synth.gaussian_kernel(shape, 2.4, 5.4, side_lobe=0.1)
+    g = gpu_time(lambda: api.BlindMaximumLikelihoodDeconvolve(obs, k0, 100), reps=1)
+    cb = cpu_time(lambda: oracle.blind_ml(obs, k0, 1))
+    emit("blind ml as written 100 iterations 256x256x64", g, 100 * cb, 0,
+         "cpu: 100 x the cost of a 1-iteration call")
     g = gpu_time(lambda: api.BlindRL(obs, k0, 4, 5), reps=1)
     cb = cpu_time(lambda: oracle.blind_rl(obs, k0, 1, 1))
     emit("blind rl 4 outer x 5 inner 256x256x64", g, 20 * cb, 0,

@@ -178,6 +178,31 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
   Work w;
   CL_TRY(work_geometry(w, in, id, kernel, kd, out));
   hipStream_t s = rt().stream;
+  // h is never updated, so renormalising it every iteration leaves it (to
+  // float rounding) as after the first: RL without energy rescale with the
+  // normalised kernel, on the fused engine where it applies
+  if (!std::getenv("CLARITY_FORCE_GENERIC")) {
+    std::unique_ptr<RLEngine> eng = make_fast_rl_unscaled(w);
+    if (eng) {
+      DevBuf obs, h, o, parts;
+      CL_TRY(obs.alloc(sizeof(float) * w.IV));
+      CL_TRY(o.alloc(sizeof(float) * w.IV));
+      CL_TRY(h.alloc(sizeof(float) * w.KV));
+      CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
+      CL_HIP(hipMemcpyAsync(obs.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+      CL_HIP(hipMemcpyAsync(h.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+      CL_TRY(k_normalize_f32sum(h.f(), w.KV, parts.d(), kParts,
+                                reinterpret_cast<float *>(parts.d() + kParts), s));
+      CL_TRY(eng->allocate());
+      CL_TRY(eng->set_inputs(obs.f(), h.f(), s));
+      CL_TRY(eng->reset(nullptr, s));
+      CL_TRY(eng->iterate(iterations, s));
+      CL_TRY(eng->estimate(o.f(), s));
+      CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+      CL_HIP(hipStreamSynchronize(s));
+      return CLARITY_SUCCESS;
+    }
+  }
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf hft, spec, o, tmp, parts;
   CL_TRY(hft.alloc(sizeof(float2) * w.C));

@@ -1835,7 +1835,10 @@ class FastRL : public RLEngine {
   // convolution and u' = max(0, u + gamma (obs - H (*) u)); A and 1/A^2 are
   // set after set_inputs (the first two scalar slots, unused by JVC).
   void make_jvc() { jvc_ = true; }
+  // RL without the energy rescale (make_fast_rl_unscaled).
+  void make_unscaled() { norescale_ = true; }
   bool is_jvc() const { return jvc_; }
+  bool is_unscaled() const { return norescale_; }
   ClarityResult_t set_jvc_constants(float A, float invASq, hipStream_t s) override {
     CL_TRY(k_fill(scal_.f(), 1, A, s));
     return k_fill(scal_.f() + 1, 1, invASq, s);
@@ -1983,7 +1986,8 @@ class FastRL : public RLEngine {
       CL_TRY(run(KY_INV, s));
       CL_TRY(run(half == 0 ? KX_MID : chained ? KX_LAST_FWD : KX_LAST, s));
     }
-    return run(KFINISH, s);
+    // without rescale the scale c stays at reset's 1: no finish
+    return norescale_ ? CLARITY_SUCCESS : run(KFINISH, s);
   }
 
   ClarityResult_t estimate(float *out_d, hipStream_t s) override {
@@ -2226,6 +2230,7 @@ class FastRL : public RLEngine {
   bool unit_ = false;      // kernel step: renormalise to sum 1
   bool flip_now_ = false;  // the z_mod being enqueued takes the sign
   bool jvc_ = false;       // Jansen-van Cittert iterations
+  bool norescale_ = false; // RL without the energy rescale
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_next_ = 0;
@@ -2509,11 +2514,19 @@ std::unique_ptr<RLEngine> make_fast_jvc(const Work &ref) {
   return std::unique_ptr<RLEngine>(e);
 }
 
+// RL without the energy rescale on the fused engine (its own alias-free
+// domain), for the as-written blind ML, whose image update has none.
+std::unique_ptr<RLEngine> make_fast_rl_unscaled(const Work &ref) {
+  std::unique_ptr<RLEngine> e = make_fast_rl(ref);
+  if (e) static_cast<FastRL *>(e.get())->make_unscaled();
+  return e;
+}
+
 std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines) {
   std::vector<FastRL *> f;
   for (RLEngine *e : engines) {
     auto *x = dynamic_cast<FastRL *>(e);
-    if (!x || x->is_jvc()) return nullptr;
+    if (!x || x->is_jvc() || x->is_unscaled()) return nullptr;
     f.push_back(x);
   }
   if (f.empty() || f.size() > 65535) return nullptr;

@@ -72,6 +72,10 @@ std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &geom);
 // domain); nullptr when the domain is not one of the fused engine's sizes.
 std::unique_ptr<RLEngine> make_fast_jvc(const Work &geom);
 
+// RL without the energy rescale on the fused engine (the estimate's scale
+// stays 1); nullptr when the shape is not the fused engine's.
+std::unique_ptr<RLEngine> make_fast_rl_unscaled(const Work &geom);
+
 // RL iterations of several engines together (for each pass, one launch per
 // kernel instantiation over all the volumes that use it).
 class RLGroup {
