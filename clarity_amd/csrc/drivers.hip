@@ -77,11 +77,39 @@ namespace {
 
 // reference Clarity_Convolve -> Clarity_ConvolveInternal
 // (src/Convolve.cxx:41-115,146-184)
+// One convolution (make_fast_convolve) or Wiener filter (make_fast_wiener)
+// of `in` on the fused passes, the image-sized result to `out`; *done =
+// false when the shapes are not the fused engine's.
+ClarityResult_t single_fused(const Work &w, const float *in, const float *kernel, float *out,
+                             bool wiener, float eps, hipStream_t s, bool *done) {
+  *done = false;
+  if (std::getenv("CLARITY_FORCE_GENERIC")) return CLARITY_SUCCESS;
+  std::unique_ptr<RLEngine> eng = wiener ? make_fast_wiener(w, eps) : make_fast_convolve(w);
+  if (!eng) return CLARITY_SUCCESS;
+  *done = true;
+  DevBuf img, ker;
+  CL_TRY(img.alloc(sizeof(float) * w.IV));
+  CL_TRY(ker.alloc(sizeof(float) * w.KV));
+  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(eng->allocate());
+  CL_TRY(eng->set_inputs(img.f(), ker.f(), s));
+  CL_TRY(eng->reset(nullptr, s));
+  CL_TRY(eng->iterate(1, s));
+  CL_TRY(eng->estimate(img.f(), s));
+  CL_HIP(hipMemcpyAsync(out, img.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
 ClarityResult_t convolve(float *in, Clarity_Dim3 id, float *kernel,
                          Clarity_Dim3 kd, float *out) {
   Work w;
   CL_TRY(work_geometry(w, in, id, kernel, kd, out));
   hipStream_t s = rt().stream;
+  bool fused = false;
+  CL_TRY(single_fused(w, in, kernel, out, false, 0.f, s, &fused));
+  if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf a, b, res;
   CL_TRY(a.alloc(sizeof(float2) * w.C));
@@ -100,6 +128,9 @@ ClarityResult_t wiener(float *in, Clarity_Dim3 id, float *kernel,
   Work w;
   CL_TRY(work_geometry(w, in, id, kernel, kd, out));
   hipStream_t s = rt().stream;
+  bool fused = false;
+  CL_TRY(single_fused(w, in, kernel, out, true, eps, s, &fused));
+  if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf a, b, res;
   CL_TRY(a.alloc(sizeof(float2) * w.C));

@@ -953,7 +953,8 @@ __device__ __forceinline__ float jvc_update(float u, float in, float o, float A,
 
 // MODE 0: the RL update u' = (c u) s2 and its sum; MODE 1: the JVC update
 // of u with the blurred estimate s2 and the observation row (cptr: A and
-// 1/A^2), no sum.
+// 1/A^2), no sum; MODE 2: u' = s2, the pass's C2R itself (convolution and
+// Wiener output).
 template <int M, int E, int TP, int HE, int MODE = 0>
 __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ S, float *__restrict__ u,
              const float *__restrict__ cptr, double *__restrict__ partials,
@@ -992,6 +993,16 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
   fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  if constexpr (MODE == 2) {
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int n0 = 2 * (t + T * e);
+      if (valid && in_image<HE>(e, n0, g.ix))
+        store_row_pair(urow, n0, g.ix, v[e].x,
+                       second_in_image<HE>(e, n0, g.ix) ? v[e].y : 0.f);
+    }
+    return;
+  }
   if constexpr (MODE == 1) {
     const float A = cptr[0], invASq = cptr[1];
 #pragma unroll
@@ -1160,6 +1171,15 @@ __global__ void __launch_bounds__(TP *(M / E))
   x_last_fwd_body<M, E, TP, HE, 1>(blockIdx.x, S, u, ap, nullptr, gtwM, gtwN, obs, g);
 }
 
+// The last x pass of a single convolution or Wiener filter: the C2R
+// written to u as it is.
+template <int M, int E, int TP, int HE>
+__global__ void __launch_bounds__(TP *(M / E))
+    k_x_out(const float2 *__restrict__ S, float *__restrict__ u,
+            const float2 *__restrict__ gtwM, const float2 *__restrict__ gtwN, Geom g) {
+  x_last_body<M, E, TP, HE, 2>(blockIdx.x, S, u, nullptr, nullptr, gtwM, gtwN, nullptr, g);
+}
+
 // ------------------------------------------------------------- y and z
 
 // H: the image is half the work domain along y (iy = N/2, a same-size
@@ -1232,9 +1252,9 @@ __global__ void __launch_bounds__(PPW *(N / E))
 // work domain along z (iz = kz = N/2), so the planes of S read and written
 // are exactly elements e < E/2, and the kernel's are e < E/4 (planes
 // z < N/4) and e >= 3E/4 (planes z >= 3N/4): no bounds tests.
-template <int N, int E, int TP, bool H>
+template <int N, int E, int TP, bool H, bool WIENER = false>
 __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, int flip) {
+        const float2 *__restrict__ gtw, Geom g, int flip, float nn2 = 0.f, float eps = 0.f) {
   static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
   constexpr int T = N / E;
   constexpr int NT = TP * T;
@@ -1298,10 +1318,22 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
   constexpr int ZV = H ? kZeroUpper : 0, ZH = H ? kZeroMiddle : 0, KO = H ? kKeepLower : 0;
   fft_pruned<N, E, 1, -1, ZV, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
   fft_pruned<N, E, 1, -1, ZH, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
-  // reference ComplexMultiplyAndScale (a c - b d, a d + b c), its scale
-  // folded into K
+  if constexpr (WIENER) {
+    // reference Wiener filter (src/WienerDeconvolve.cxx:54-64): Hc =
+    // conj(H) (1 / (|H|^2 + eps)), then Hc G s.  K holds s H (s = 1/n, n
+    // = nx ny nz), so Hc s = conj(s H) / (n^2 |s H|^2 + eps)
 #pragma unroll
-  for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
+    for (int e = 0; e < E; e++) {
+      const float2 hh = h[e];
+      const float inv = 1.0f / (nn2 * (hh.x * hh.x + hh.y * hh.y) + eps);
+      v[e] = cmul(v[e], make_float2(hh.x * inv, -hh.y * inv));
+    }
+  } else {
+    // reference ComplexMultiplyAndScale (a c - b d, a d + b c), its scale
+    // folded into K
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
+  }
   if (flip) {
     // the product also times (-1)^(kx + ky + kz), a cyclic shift by half
     // the work domain on every axis.  After the forward transforms element
@@ -1338,6 +1370,13 @@ __global__ void __launch_bounds__(TP *(N / E))
     kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
   z_body<N, E, TP, H>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.flip);
+}
+// z pass of the Wiener filter (single volume)
+template <int N, int E, int TP, bool H>
+__global__ void __launch_bounds__(TP *(N / E))
+    k_zw(float2 *__restrict__ S, const float2 *__restrict__ K, const float2 *__restrict__ gtw,
+         Geom g, float nn2, float eps) {
+  z_body<N, E, TP, H, true>(blockIdx.x, S, K, gtw, g, 0, nn2, eps);
 }
 
 // The same-size z pass (iz = kz = N/2) over several column tiles per
@@ -1440,8 +1479,9 @@ __global__ void __launch_bounds__(TP *(N / E)) __attribute__((amdgpu_waves_per_e
 
 enum Kind { KX_FWD, KY_FWD, KZ_MOD, KY_INV, KX_MID, KX_LAST, KFINISH, KX_LAST_FWD, KNUM,
             // the Jansen-van Cittert engine's last x passes (timed as x_last
-            // and x_last_fwd)
-            KX_JVC, KX_JVC_FWD };
+            // and x_last_fwd), and the output pass of a single convolution
+            // or Wiener filter (timed as x_last)
+            KX_JVC, KX_JVC_FWD, KX_OUT };
 const char *const kNames[KNUM] = {"x_fwd", "y_fwd", "z_mod",  "y_inv",
                                   "x_mid", "x_last", "finish", "x_last_fwd"};
 
@@ -1524,6 +1564,9 @@ void launch_x_t(Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   } else if (kind == KX_JVC_FWD) {
     hipLaunchKernelGGL((k_x_jvc_fwd<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u,
                        b.scal, b.twM, b.twN, b.obs, g);
+  } else if (kind == KX_OUT) {
+    hipLaunchKernelGGL((k_x_out<M, E, TP, HE>), dim3(grid), dim3(NT), 0, s, b.S, b.u, b.twM,
+                       b.twN, g);
   }
 }
 
@@ -1568,7 +1611,7 @@ void launch_x(int e, int tp, bool h, Kind kind, const Geom &g, const Bufs &b,
 
 void dispatch_x(const Tune &tu, Kind kind, const Geom &g, const Bufs &b, hipStream_t s) {
   const int j = (kind == KX_FWD) ? 0 : (kind == KX_MID) ? 1
-                : (kind == KX_LAST || kind == KX_JVC) ? 2 : 3;
+                : (kind == KX_LAST || kind == KX_JVC || kind == KX_OUT) ? 2 : 3;
   const int jt = j < 3 ? j : 2;  // x_last_fwd takes x_last's launch shape
   const int e = tu.xe[jt], tp = tu.xt[jt];
   const bool h = (tu.xh >> j) & 1;  // the launcher checks that the rows allow it
@@ -1637,14 +1680,32 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
   }
 }
 
+// What z_mod does with the spectrum column: the product with the OTF,
+// with the kernel step's sign (flip), or the Wiener filter.
+struct ZOp {
+  bool flip = false;
+  bool wiener = false;
+  float nn2 = 0.f, eps = 0.f;  // Wiener: (nx ny nz)^2 and epsilon
+};
+
 template <int N, int E, int TP>
-void launch_z_t(int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z_t(int zw, const ZOp &op, const Geom &g, const Bufs &b, hipStream_t s) {
+  const bool flip = op.flip;
   const int ntot = g.hx * (g.ny / TP);
   const unsigned grid = (unsigned)ntot;
   // the pipelined kernel's 32-bit buffer offsets: extent plus N/2 - T
   // planes (its out-of-range offset plus the largest element offset)
   const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
   const bool fits32 = 2 * extent < (1ull << 32);
+  if (op.wiener) {
+    if (2 * g.iz == N && 2 * g.kz == N)
+      hipLaunchKernelGGL((k_zw<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
+                         b.K, b.twZ, g, op.nn2, op.eps);
+    else
+      hipLaunchKernelGGL((k_zw<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
+                         b.K, b.twZ, g, op.nn2, op.eps);
+    return;
+  }
   if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32 && !flip && !g_rec)
     hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
                        dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
@@ -1662,40 +1723,40 @@ void launch_z_t(int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
 // its tile within 128 KB of LDS, and ny is a multiple of the width.
 template <int N, int E>
-void launch_z_e(int tp, int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z_e(int tp, int zw, const ZOp &op, const Geom &g, const Bufs &b, hipStream_t s) {
   constexpr bool ok32 = 32 * (N / E) <= 1024 && 32 * N * 8 <= 131072;
   constexpr bool ok64 = 64 * (N / E) <= 1024 && 64 * N * 8 <= 131072;
   if constexpr (ok64) {
-    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(zw, flip, g, b, s);
+    if (tp == 64 && g.ny % 64 == 0) return launch_z_t<N, E, 64>(zw, op, g, b, s);
   }
   if constexpr (ok32) {
-    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(zw, flip, g, b, s);
+    if (tp >= 32 && g.ny % 32 == 0) return launch_z_t<N, E, 32>(zw, op, g, b, s);
   }
-  launch_z_t<N, E, kTPz>(zw, flip, g, b, s);
+  launch_z_t<N, E, kTPz>(zw, op, g, b, s);
 }
 
 template <int N>
-void launch_z(int e, int tp, int zw, bool flip, const Geom &g, const Bufs &b, hipStream_t s) {
+void launch_z(int e, int tp, int zw, const ZOp &op, const Geom &g, const Bufs &b, hipStream_t s) {
   if constexpr (N % 3 == 0) {
-    launch_z_e<N, 12>(tp, zw, flip, g, b, s);
+    launch_z_e<N, 12>(tp, zw, op, g, b, s);
   } else if constexpr (N == 64) {
-    launch_z_e<N, 8>(tp, zw, flip, g, b, s);
+    launch_z_e<N, 8>(tp, zw, op, g, b, s);
   } else {
     if (e == 8)
-      launch_z_e<N, 8>(tp, zw, flip, g, b, s);
+      launch_z_e<N, 8>(tp, zw, op, g, b, s);
     else
-      launch_z_e<N, 16>(tp, zw, flip, g, b, s);
+      launch_z_e<N, 16>(tp, zw, op, g, b, s);
   }
 }
 
-void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, hipStream_t s, bool flip) {
+void dispatch_z(const Tune &tu, const Geom &g, const Bufs &b, hipStream_t s, const ZOp &op) {
   switch (g.nz) {
-    case 64: launch_z<64>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
-    case 96: launch_z<96>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
-    case 128: launch_z<128>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
-    case 192: launch_z<192>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
-    case 256: launch_z<256>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
-    case 512: launch_z<512>(tu.ze, tu.zt, tu.zw, flip, g, b, s); break;
+    case 64: launch_z<64>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
+    case 96: launch_z<96>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
+    case 128: launch_z<128>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
+    case 192: launch_z<192>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
+    case 256: launch_z<256>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
+    case 512: launch_z<512>(tu.ze, tu.zt, tu.zw, op, g, b, s); break;
   }
 }
 
@@ -1816,7 +1877,12 @@ class FastRL : public RLEngine {
       case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, nullptr); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, nullptr); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, nullptr); break;
-      case KZ_MOD: dispatch_z(tu_, g_, b_, nullptr, flip_ && second); break;
+      case KZ_MOD: {
+        ZOp op;
+        op.flip = flip_ && second;
+        dispatch_z(tu_, g_, b_, nullptr, op);
+        break;
+      }
       default: break;
     }
     g_rec = nullptr;
@@ -1837,6 +1903,16 @@ class FastRL : public RLEngine {
   void make_jvc() { jvc_ = true; }
   // RL without the energy rescale (make_fast_rl_unscaled).
   void make_unscaled() { norescale_ = true; }
+  // One convolution of the estimate (the input, by reset), or the Wiener
+  // filter of it: iterate(1) writes the result to the estimate.
+  void make_output(bool wiener, float eps) {
+    out_ = true;
+    wiener_ = wiener;
+    eps_ = eps;
+    const double n = (double)g_.nx * g_.ny * g_.nz;
+    nn2_ = (float)(n * n);
+  }
+  bool is_output() const { return out_; }
   bool is_jvc() const { return jvc_; }
   bool is_unscaled() const { return norescale_; }
   ClarityResult_t set_jvc_constants(float A, float invASq, hipStream_t s) override {
@@ -1926,7 +2002,9 @@ class FastRL : public RLEngine {
   ClarityResult_t iterate(int n, hipStream_t s) override {
     if (n <= 0) return CLARITY_SUCCESS;
     const bool chain = tu_.fx != 0;
-    if (timing_ || !use_graph_) {
+    // a single convolution or filter is launched directly (capturing it
+    // would cost more than the launches)
+    if (timing_ || !use_graph_ || out_) {
       for (int it = 0; it < n; it++) {
         if (it == 0 || !chain) CL_TRY(run(KX_FWD, s));
         CL_TRY(enqueue_body(chain && it + 1 < n, s));
@@ -1970,6 +2048,13 @@ class FastRL : public RLEngine {
   // One iteration after its x_fwd: nine launches (the last x pass fused
   // with the next iteration's x_fwd when `chained`).
   ClarityResult_t enqueue_body(bool chained, hipStream_t s) {
+    if (out_) {
+      // one convolution (or Wiener filter), its C2R the result
+      CL_TRY(run(KY_FWD, s));
+      CL_TRY(run(KZ_MOD, s));
+      CL_TRY(run(KY_INV, s));
+      return run(KX_OUT, s);
+    }
     if (jvc_) {
       // one convolution, then the JVC update in the last x pass
       CL_TRY(run(KY_FWD, s));
@@ -2201,10 +2286,19 @@ class FastRL : public RLEngine {
       case KX_LAST:
       case KX_LAST_FWD:
       case KX_JVC:
-      case KX_JVC_FWD: dispatch_x(tu_, kind, g_, b_, s); break;
+      case KX_JVC_FWD:
+      case KX_OUT: dispatch_x(tu_, kind, g_, b_, s); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, s); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
-      case KZ_MOD: dispatch_z(tu_, g_, b_, s, flip_now_); break;
+      case KZ_MOD: {
+        ZOp op;
+        op.flip = flip_now_;
+        op.wiener = wiener_;
+        op.nn2 = nn2_;
+        op.eps = eps_;
+        dispatch_z(tu_, g_, b_, s, op);
+        break;
+      }
       case KFINISH:
         CL_TRY(k_finish_sum(b_.parts, nparts_, b_.scal + 1, b_.scal, b_.scal + 2, s));
         break;
@@ -2213,7 +2307,8 @@ class FastRL : public RLEngine {
     CL_TRY(launch_check());
     if (timing_) {
       CL_HIP(hipEventRecord(ev_pool_[slot].second, s));
-      pending_.push_back({kind == KX_JVC ? KX_LAST : kind == KX_JVC_FWD ? KX_LAST_FWD : kind,
+      pending_.push_back({(kind == KX_JVC || kind == KX_OUT) ? KX_LAST
+                          : kind == KX_JVC_FWD ? KX_LAST_FWD : kind,
                           slot});
     }
     return CLARITY_SUCCESS;
@@ -2231,6 +2326,9 @@ class FastRL : public RLEngine {
   bool flip_now_ = false;  // the z_mod being enqueued takes the sign
   bool jvc_ = false;       // Jansen-van Cittert iterations
   bool norescale_ = false; // RL without the energy rescale
+  bool out_ = false;       // one convolution / Wiener filter
+  bool wiener_ = false;
+  float nn2_ = 0.f, eps_ = 0.f;
   bool timing_ = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
   size_t ev_next_ = 0;
@@ -2522,11 +2620,35 @@ std::unique_ptr<RLEngine> make_fast_rl_unscaled(const Work &ref) {
   return e;
 }
 
+// A single convolution on the fused passes: RL's first convolution, so
+// over the same alias-free domain (the output is the image corner).
+std::unique_ptr<RLEngine> make_fast_convolve(const Work &ref) {
+  std::unique_ptr<RLEngine> e = make_fast_rl(ref);
+  if (e) static_cast<FastRL *>(e.get())->make_output(false, 0.f);
+  return e;
+}
+
+// The Wiener filter on the fused passes, over the reference domain: the
+// filter conj(H) / (|H|^2 + eps) has no compact support, so no smaller
+// domain gives the reference's result.
+std::unique_ptr<RLEngine> make_fast_wiener(const Work &ref, float eps) {
+  Work geom;
+  geom.set_geometry(ref);
+  geom.wd = ref.wd;
+  geom.V = geom.wd.voxels();
+  geom.C = (long long)geom.wd.z * geom.wd.y * (geom.wd.x / 2 + 1);
+  geom.scale = inv_volume(geom.wd.x, geom.wd.y, geom.wd.z);
+  if (!FastRL::supports(geom)) return nullptr;
+  auto *e = new FastRL(geom);
+  e->make_output(true, eps);
+  return std::unique_ptr<RLEngine>(e);
+}
+
 std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines) {
   std::vector<FastRL *> f;
   for (RLEngine *e : engines) {
     auto *x = dynamic_cast<FastRL *>(e);
-    if (!x || x->is_jvc() || x->is_unscaled()) return nullptr;
+    if (!x || x->is_jvc() || x->is_unscaled() || x->is_output()) return nullptr;
     f.push_back(x);
   }
   if (f.empty() || f.size() > 65535) return nullptr;

@@ -76,6 +76,12 @@ std::unique_ptr<RLEngine> make_fast_jvc(const Work &geom);
 // stays 1); nullptr when the shape is not the fused engine's.
 std::unique_ptr<RLEngine> make_fast_rl_unscaled(const Work &geom);
 
+// A single convolution (alias-free domain) or Wiener filter (reference
+// domain) on the fused passes: reset(input), iterate(1), estimate(out);
+// nullptr when the shape is not the fused engine's.
+std::unique_ptr<RLEngine> make_fast_convolve(const Work &geom);
+std::unique_ptr<RLEngine> make_fast_wiener(const Work &geom, float eps);
+
 // RL iterations of several engines together (for each pass, one launch per
 // kernel instantiation over all the volumes that use it).
 class RLGroup {

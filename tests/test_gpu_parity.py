@@ -127,9 +127,23 @@ def test_clip_bit_exact(clarity_gpu):
 
 # ---------------------------------------------------------------- drivers
 
-def test_convolve_vs_oracle_small(clarity_gpu, small):
+def _engine(engine, fn):
+    """fn() on the fused passes (the default where they apply) or, with
+    engine == "generic", on the generic driver (CLARITY_FORCE_GENERIC)."""
+    if engine == "generic":
+        os.environ["CLARITY_FORCE_GENERIC"] = "1"
+    try:
+        return fn()
+    finally:
+        os.environ.pop("CLARITY_FORCE_GENERIC", None)
+
+
+@pytest.mark.parametrize("engine", ["fused", "generic"])
+def test_convolve_vs_oracle_small(clarity_gpu, small, engine):
+    """Convolve on the fused passes (RL's first convolution, over its
+    alias-free domain) and on the generic driver."""
     api = clarity_gpu
-    got = api.Convolve(small.obj, small.psf)
+    got = _engine(engine, lambda: api.Convolve(small.obj, small.psf))
     ref = oracle.convolve(small.obj, small.psf)
     assert rel_l2(got, ref) <= 1e-5
     # the oracle itself against numpy float64 (independent FFT)
@@ -158,11 +172,24 @@ def test_convolve_delta_identity(clarity_gpu):
     assert rel_l2(api.Convolve(a, k), a) <= 1e-6
 
 
-def test_wiener_vs_oracle(clarity_gpu, small):
+@pytest.mark.parametrize("engine", ["fused", "generic"])
+@pytest.mark.parametrize("name", ["beads64", "96x64x48"])
+def test_wiener_vs_oracle(clarity_gpu, small, name, engine):
+    """Wiener on the fused passes (the filter in z_mod, over the reference
+    domain: 128 x 128 x 64 and, radix 3, 192 x 128 x 96) and on the generic
+    driver."""
     api = clarity_gpu
-    got = api.WienerDeconvolve(small.obs, small.psf, 1e-3)
-    ref = oracle.wiener(small.obs, small.psf, 1e-3)
-    assert rel_l2(got, ref) <= 1e-5
+    if name == "beads64":
+        obs, psf = small.obs, small.psf
+    else:
+        case = synth.make_workload("beads-64x64x32", seed=6, shape=(48, 64, 96))
+        obs, psf = case.obs, case.psf
+    for eps in (1e-3, 1e-1):
+        got = _engine(engine, lambda: api.WienerDeconvolve(obs, psf, eps))
+        ref = oracle.wiener(obs, psf, eps)
+        err = rel_l2(got, ref)
+        print(f"[wiener] {name} {engine} eps {eps}: rel L2 {err:.2e}", flush=True)
+        assert err <= 1e-5, eps
 
 
 def test_wiener_delta_kernel(clarity_gpu):
