@@ -1248,14 +1248,17 @@ __global__ void __launch_bounds__(PPW *(N / E))
 // column of S is multiplied by the matching OTF column, the z transform of
 // the kernel column of K, made here in registers (the same FFT as the
 // spectrum's; the kernel, shifted by -kz/2, has its kz nonzero planes at
-// z < ka and z >= nz - (kz - ka)).  H: image and kernel are both half the
-// work domain along z (iz = kz = N/2), so the planes of S read and written
-// are exactly elements e < E/2, and the kernel's are e < E/4 (planes
-// z < N/4) and e >= 3E/4 (planes z >= 3N/4): no bounds tests.
-template <int N, int E, int TP, bool H, bool WIENER = false>
+// z < ka and z >= nz - (kz - ka)).  HZ > 0: image and kernel have the
+// same depth iz = kz = HZ * T, a whole number of elements per thread (the
+// launcher uses HZ = E/2: half the work domain), so the planes of S read
+// and written are exactly elements e < HZ, and the kernel's are e < HZ/2
+// (planes z < kz/2) and e >= E - HZ/2 (planes z >= nz - kz/2): no bounds
+// tests.
+template <int N, int E, int TP, int HZ, bool WIENER = false>
 __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ K,
         const float2 *__restrict__ gtw, Geom g, int flip, float nn2 = 0.f, float eps = 0.f) {
-  static_assert(!H || E % 4 == 0, "kernel planes in whole elements");
+  constexpr bool H = HZ > 0;
+  static_assert(HZ % 2 == 0 && HZ <= E, "kernel planes in whole elements");
   constexpr int T = N / E;
   constexpr int NT = TP * T;
   __shared__ float2 lds[TP * N];
@@ -1280,7 +1283,7 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
 #pragma unroll
     for (int e = 0; e < E; e++, p += step) {
       if constexpr (H)
-        v[e] = (e < E / 2) ? *p : make_float2(0.f, 0.f);
+        v[e] = (e < HZ) ? *p : make_float2(0.f, 0.f);
       else
         v[e] = (t + T * e < g.iz) ? *p : make_float2(0.f, 0.f);
     }
@@ -1289,11 +1292,12 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
 #pragma unroll
   for (int e = 0; e < E; e++) {
     if constexpr (H) {
-      // plane z = t + T*e; upper planes sit N/2 = (E/2)*T planes lower in K
-      if (e < E / 4)
+      // plane z = t + T*e; upper planes sit nz - kz = (E - HZ)*T planes
+      // lower in K
+      if (e < HZ / 2)
         h[e] = k0[e * step];
-      else if (e >= 3 * E / 4)
-        h[e] = k0[(e - E / 2) * step];
+      else if (e >= E - HZ / 2)
+        h[e] = k0[(e - (E - HZ)) * step];
       else
         h[e] = make_float2(0.f, 0.f);
     } else {
@@ -1313,9 +1317,12 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
   twr.store(tw);
   __syncthreads();
   const InterLds<TP> L{lds + q};
-  // H: the spectrum column is zero in its upper half, the kernel column in
-  // its middle half, and the inverse is stored for its lower half
-  constexpr int ZV = H ? kZeroUpper : 0, ZH = H ? kZeroMiddle : 0, KO = H ? kKeepLower : 0;
+  // HZ = E/2: the spectrum column is zero in its upper half, the kernel
+  // column in its middle half, and the inverse is stored for its lower
+  // half (patterns the first and last butterflies prune)
+  constexpr bool HALF = 2 * HZ == E;
+  constexpr int ZV = HALF ? kZeroUpper : 0, ZH = HALF ? kZeroMiddle : 0,
+                KO = HALF ? kKeepLower : 0;
   fft_pruned<N, E, 1, -1, ZV, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
   fft_pruned<N, E, 1, -1, ZH, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
   if constexpr (WIENER) {
@@ -1351,7 +1358,7 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
 #pragma unroll
     for (int e = 0; e < E; e++, p += step) {
       if constexpr (H) {
-        if (e < E / 2) *p = v[e];
+        if (e < HZ) *p = v[e];
       } else {
         if (t + T * e < g.iz) *p = v[e];
       }
@@ -1359,24 +1366,24 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
   }
 }
 
-template <int N, int E, int TP, bool H>
+template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
         const float2 *__restrict__ gtw, Geom g, int flip) {
-  z_body<N, E, TP, H>(blockIdx.x, S, K, gtw, g, flip);
+  z_body<N, E, TP, HZ>(blockIdx.x, S, K, gtw, g, flip);
 }
-template <int N, int E, int TP, bool H>
+template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
-  z_body<N, E, TP, H>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.flip);
+  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.flip);
 }
 // z pass of the Wiener filter (single volume)
-template <int N, int E, int TP, bool H>
+template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     k_zw(float2 *__restrict__ S, const float2 *__restrict__ K, const float2 *__restrict__ gtw,
          Geom g, float nn2, float eps) {
-  z_body<N, E, TP, H, true>(blockIdx.x, S, K, gtw, g, 0, nn2, eps);
+  z_body<N, E, TP, HZ, true>(blockIdx.x, S, K, gtw, g, 0, nn2, eps);
 }
 
 // The same-size z pass (iz = kz = N/2) over several column tiles per
@@ -1688,36 +1695,34 @@ struct ZOp {
   float nn2 = 0.f, eps = 0.f;  // Wiener: (nx ny nz)^2 and epsilon
 };
 
+template <int N, int E, int TP, int HZ>
+void launch_z_hz(const ZOp &op, const Geom &g, const Bufs &b, unsigned grid, hipStream_t s) {
+  if (op.wiener) {
+    hipLaunchKernelGGL((k_zw<N, E, TP, HZ>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.K,
+                       b.twZ, g, op.nn2, op.eps);
+    return;
+  }
+  if (recorded(&kb_z<N, E, TP, HZ>, grid, TP * (N / E))) return;
+  hipLaunchKernelGGL((k_z<N, E, TP, HZ>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.K,
+                     b.twZ, g, (int)op.flip);
+}
+
 template <int N, int E, int TP>
 void launch_z_t(int zw, const ZOp &op, const Geom &g, const Bufs &b, hipStream_t s) {
-  const bool flip = op.flip;
   const int ntot = g.hx * (g.ny / TP);
   const unsigned grid = (unsigned)ntot;
   // the pipelined kernel's 32-bit buffer offsets: extent plus N/2 - T
   // planes (its out-of-range offset plus the largest element offset)
   const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
   const bool fits32 = 2 * extent < (1ull << 32);
-  if (op.wiener) {
-    if (2 * g.iz == N && 2 * g.kz == N)
-      hipLaunchKernelGGL((k_zw<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                         b.K, b.twZ, g, op.nn2, op.eps);
-    else
-      hipLaunchKernelGGL((k_zw<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                         b.K, b.twZ, g, op.nn2, op.eps);
-    return;
-  }
-  if (2 * g.iz == N && 2 * g.kz == N && zw > 1 && fits32 && !flip && !g_rec)
+  const bool half = 2 * g.iz == N && 2 * g.kz == N;
+  if (half && zw > 1 && fits32 && !op.flip && !op.wiener && !g_rec) {
     hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
                        dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
-  else if (2 * g.iz == N && 2 * g.kz == N) {
-    if (recorded(&kb_z<N, E, TP, true>, grid, TP * (N / E))) return;
-    hipLaunchKernelGGL((k_z<N, E, TP, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g, (int)flip);
-  } else {
-    if (recorded(&kb_z<N, E, TP, false>, grid, TP * (N / E))) return;
-    hipLaunchKernelGGL((k_z<N, E, TP, false>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
-                       b.K, b.twZ, g, (int)flip);
+    return;
   }
+  if (half) return launch_z_hz<N, E, TP, E / 2>(op, g, b, grid, s);
+  launch_z_hz<N, E, TP, 0>(op, g, b, grid, s);
 }
 
 // Tile widths 32 and 64 where the workgroup stays within 1024 threads and
