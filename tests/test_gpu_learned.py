@@ -78,7 +78,7 @@ def test_gate_falls_back_to_plain_rl(clarity_gpu, trained):
     learned = plan.estimate()
     plan.close()
     ev = card["evaluation"]
-    # the card's own verdict at the acceptance bar
+    # the card's own decision at the acceptance bar
     est, path = L.deconvolve(p.obs, p.psf, p.sigma_xy, p.sigma_z)
     assert path == ("learned" if ev["accepted"] else "plain")
     if path == "plain":
