@@ -1960,9 +1960,12 @@ class FastRL : public RLEngine {
     // instead of nz puts work planes z < ka at p = z and the planes
     // z >= nz - kz/2 at p = z - (nz - kz), the layout z_mod reads.
     {
-      DevBuf pad;
+      // the padded kernel's buffer is kept by the engine (nx ny kz floats),
+      // so set-up does not wait for the transform to free it; calls on the
+      // stream reuse it in order
       const Dims pd{g_.nx, g_.ny, g_.kz};
-      CL_TRY(pad.alloc(sizeof(float) * (size_t)pd.voxels()));
+      DevBuf &pad = pad_;
+      if (!pad.p) CL_TRY(pad.alloc(sizeof(float) * (size_t)pd.voxels()));
       const int kshift[3] = {-w_.kd.x / 2, -w_.kd.y / 2, -w_.kd.z / 2};
       CL_TRY(k_pad_shift(pad.f(), pd, psf_d, w_.kd, kshift, 0.f, s));
       // the product's 1/(nx ny nz) (the reference's ComplexMultiplyAndScale
@@ -1980,7 +1983,6 @@ class FastRL : public RLEngine {
       dispatch_x(tu_, KX_FWD, go, bo, s);
       dispatch_y(tu_, true, go, bo, s);
       CL_TRY(launch_check());
-      CL_HIP(hipStreamSynchronize(s));  // pad is released on return
     }
     // E_obs: the padded observation's sum (zeros outside the corner); the
     // kernel step's target is 1 instead (the estimate renormalised to sum 1)
@@ -2325,7 +2327,7 @@ class FastRL : public RLEngine {
   hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};  // see capture()
   Bufs b_{};
   int nparts_ = 0;
-  DevBuf obs_, u_, S_, K_, parts_, scal_;
+  DevBuf obs_, u_, S_, K_, parts_, scal_, pad_;
   bool flip_ = false;      // kernel step: the correction's spectrum sign
   bool unit_ = false;      // kernel step: renormalise to sum 1
   bool flip_now_ = false;  // the z_mod being enqueued takes the sign
