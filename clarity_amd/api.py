@@ -78,6 +78,11 @@ _SIGS = [
     ("ClarityEx_MaximumLikelihoodDeconvolveFrom", _R,
      [_F, Dim3, _F, Dim3, _F, _F, ctypes.c_int]),
     ("ClarityEx_BlindRL", _R, [_F, Dim3, _F, Dim3, _F, _F, ctypes.c_int, ctypes.c_int]),
+    ("ClarityEx_BlindRLFrom", _R,
+     [_F, Dim3, _F, Dim3, _F, _F, _F, ctypes.c_int, ctypes.c_int]),
+    ("ClarityEx_DivideArraysComponentWise", _R,
+     [_F, _F, _F, ctypes.c_float, ctypes.c_int64]),
+    ("ClarityEx_LastEngine", ctypes.c_char_p, []),
     ("ClarityEx_IDivergence", _R,
      [_F, _F, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
     ("ClarityEx_RLPlanCreate", _R, [ctypes.POINTER(ctypes.c_void_p), Dim3, Dim3]),
@@ -246,15 +251,42 @@ def MaximumLikelihoodDeconvolveBatch(images, kernels, iterations: int):
     return outs
 
 
-def BlindRL(img, kernel, outer: int, inner: int):
-    """ClarityEx_BlindRL -> (image estimate, kernel estimate)."""
+def BlindRL(img, kernel, outer: int, inner: int, initial=None):
+    """ClarityEx_BlindRL (ClarityEx_BlindRLFrom with ``initial``, the first
+    image estimate) -> (image estimate, kernel estimate)."""
     img = _vol(img)
     psf = _vol(kernel)
     out = np.empty_like(img)
     kout = np.empty_like(psf)
-    _check(lib().ClarityEx_BlindRL(_p(img), dim3(img), _p(psf), dim3(psf), _p(out),
-                                   _p(kout), outer, inner), "ClarityEx_BlindRL")
+    if initial is None:
+        _check(lib().ClarityEx_BlindRL(_p(img), dim3(img), _p(psf), dim3(psf), _p(out),
+                                       _p(kout), outer, inner), "ClarityEx_BlindRL")
+    else:
+        init = _vol(initial)
+        if init.shape != img.shape:
+            raise ValueError("initial estimate must have the image's shape")
+        _check(lib().ClarityEx_BlindRLFrom(_p(img), dim3(img), _p(psf), dim3(psf),
+                                           _p(init), _p(out), _p(kout), outer, inner),
+               "ClarityEx_BlindRLFrom")
     return out, kout
+
+
+def DivideArraysComponentWise(a, b, value: float = 0.0):
+    """ClarityEx_DivideArraysComponentWise: the fused passes' guarded ratio
+    over flat float32 arrays."""
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1)
+    if a.shape != b.shape:
+        raise ValueError("a and b must have the same size")
+    r = np.empty_like(a)
+    _check(lib().ClarityEx_DivideArraysComponentWise(_p(r), _p(a), _p(b), value, a.size),
+           "ClarityEx_DivideArraysComponentWise")
+    return r
+
+
+def LastEngine() -> str:
+    """Engine of the last single-volume driver call: "fast" or "generic"."""
+    return lib().ClarityEx_LastEngine().decode()
 
 
 def IDivergence(g, ghat) -> float:

@@ -31,9 +31,31 @@ struct Runtime {
   std::multimap<size_t, void *> cache;  // freed device blocks by size
   size_t cache_bytes = 0;
   std::map<int, float2 *> twiddles;     // w_N^j tables, j < N
+  const char *last_engine = "";         // engine of the last driver call
 };
 
 Runtime &rt();
+
+// A registration reference held by a long-lived handle (RL plan, network):
+// the runtime's stream, twiddle tables and memory cache, which the handle's
+// kernels use, stay alive until the handle is destroyed, even after the
+// caller's last Clarity_UnRegister.  Declare it as the handle's first
+// member, so that it is released after the handle's device buffers.
+struct RegHold {
+  bool held = false;
+  RegHold() = default;
+  RegHold(const RegHold &) = delete;
+  RegHold &operator=(const RegHold &) = delete;
+  ClarityResult_t acquire() {
+    if (held) return CLARITY_SUCCESS;
+    const ClarityResult_t r = Clarity_Register();
+    held = r == CLARITY_SUCCESS;
+    return r;
+  }
+  ~RegHold() {
+    if (held) (void)Clarity_UnRegister();
+  }
+};
 
 // True once Clarity_Register succeeded (and before the last UnRegister).
 bool ready();
@@ -107,6 +129,25 @@ inline bool dims_ok(Clarity_Dim3 d) { return d.x > 0 && d.y > 0 && d.z > 0; }
 // in float is |d| < kDivGuard.
 constexpr float kDivGuard = 0x1.4f8b5ap-17f;
 
+// a / b for normal b (the guarded ratio has |b| >= 1e-5): the hardware
+// reciprocal (1 ulp) and one Newton correction of the quotient, q + (a - b
+// q) / b with the residual exact in one fma; within 1 ulp of the correctly
+// rounded quotient for 1e-5 <= |b| < 2^126 in 4 instructions, where IEEE
+// division takes about 10 (scaling, reciprocal, refinement, fix-up for b
+// near the range limits).
+__device__ __forceinline__ float quotient(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return fmaf(fmaf(-b, q, a), r, q);
+}
+
+// The divide guard of the fused passes: `value` where |b| < 1e-5 (the
+// reference's double compare, src/ComputePrimitives.cxx:143-148), else
+// quotient(a, b).
+__device__ __forceinline__ float guarded_ratio(float a, float b, float value) {
+  return (fabsf(b) < kDivGuard) ? value : quotient(a, b);
+}
+
 // 1/(nz*ny*nx) as the reference computes it: ((float)nz * ny) * nx in float
 // (reference src/Convolve.cxx:199).
 inline float inv_volume(int nx, int ny, int nz) {
@@ -163,10 +204,15 @@ ClarityResult_t fft3d_c2r(int nx, int ny, int nz, float2 *spec, float *out,
 // Pointwise kernels (pointwise.hip), all on stream s, n elements.
 // ---------------------------------------------------------------------------
 
+// out = scale * a * b (conj_b: scale * a * conj(b), a correlation)
 ClarityResult_t k_modulate(const float2 *a, const float2 *b, float2 *out,
-                           long long n, float scale, hipStream_t s);
+                           long long n, float scale, hipStream_t s,
+                           bool conj_b = false);
 ClarityResult_t k_wiener(const float2 *in, const float2 *psf, float2 *out,
                          long long n, float scale, float eps, hipStream_t s);
+// r = guarded_ratio(a, b, value): the fused passes' divide
+ClarityResult_t k_guarded_divide(float *r, const float *a, const float *b,
+                                 float value, long long n, hipStream_t s);
 ClarityResult_t k_divide(float *r, const float *a, const float *b,
                          float value, long long n, hipStream_t s);
 // r = a * b, with float64 per-block partial sums of r into partials.

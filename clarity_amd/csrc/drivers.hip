@@ -57,9 +57,9 @@ ClarityResult_t work_upload(Work &w, const float *in, const float *kernel,
 }
 
 ClarityResult_t convolve_otf(const Work &w, const float *in, const float2 *otf,
-                             float *out, float2 *spec, hipStream_t s) {
+                             float *out, float2 *spec, hipStream_t s, bool conj) {
   CL_TRY(fft3d_r2c(w.wd.x, w.wd.y, w.wd.z, in, spec, s));
-  CL_TRY(k_modulate(spec, otf, spec, w.C, w.scale, s));
+  CL_TRY(k_modulate(spec, otf, spec, w.C, w.scale, s, conj));
   return fft3d_c2r(w.wd.x, w.wd.y, w.wd.z, spec, out, s);
 }
 
@@ -109,6 +109,7 @@ ClarityResult_t convolve(float *in, Clarity_Dim3 id, float *kernel,
   hipStream_t s = rt().stream;
   bool fused = false;
   CL_TRY(single_fused(w, in, kernel, out, false, 0.f, s, &fused));
+  rt().last_engine = fused ? "fast" : "generic";
   if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf a, b, res;
@@ -130,6 +131,7 @@ ClarityResult_t wiener(float *in, Clarity_Dim3 id, float *kernel,
   hipStream_t s = rt().stream;
   bool fused = false;
   CL_TRY(single_fused(w, in, kernel, out, true, eps, s, &fused));
+  rt().last_engine = fused ? "fast" : "generic";
   if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf a, b, res;
@@ -173,6 +175,7 @@ ClarityResult_t jvc(float *in, Clarity_Dim3 id, float *kernel, Clarity_Dim3 kd,
   if (!std::getenv("CLARITY_FORCE_GENERIC")) {
     std::unique_ptr<RLEngine> eng = make_fast_jvc(w);
     if (eng) {
+      rt().last_engine = "fast";
       DevBuf est;
       CL_TRY(est.alloc(sizeof(float) * w.V));
       CL_TRY(eng->allocate());
@@ -185,6 +188,7 @@ ClarityResult_t jvc(float *in, Clarity_Dim3 id, float *kernel, Clarity_Dim3 kd,
     }
   }
   ker.release();
+  rt().last_engine = "generic";
   DevBuf otf, spec, ik, ok;
   CL_TRY(otf.alloc(sizeof(float2) * w.C));
   CL_TRY(spec.alloc(sizeof(float2) * w.C));
@@ -215,6 +219,7 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
   if (!std::getenv("CLARITY_FORCE_GENERIC")) {
     std::unique_ptr<RLEngine> eng = make_fast_rl_unscaled(w);
     if (eng) {
+      rt().last_engine = "fast";
       DevBuf obs, h, o, parts;
       CL_TRY(obs.alloc(sizeof(float) * w.IV));
       CL_TRY(o.alloc(sizeof(float) * w.IV));
@@ -234,6 +239,7 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
       return CLARITY_SUCCESS;
     }
   }
+  rt().last_engine = "generic";
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf hft, spec, o, tmp, parts;
   CL_TRY(hft.alloc(sizeof(float2) * w.C));
@@ -258,13 +264,14 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
 // iterations of the kernel-fixed problem (the engine on its alias-free work
 // domain), the kernel steps RL iterations of the image-fixed problem with
 // the kernel as the estimate (make_fast_rl_kernel_step: reference work
-// domain, the correction's spectrum sign, renormalisation to sum 1).  The
-// image and kernel estimates stay in their caller layouts, each engine's
-// estimate the other's blurring input.  *done = false when the shapes are
+// domain, the correction a correlation with the image, renormalisation to
+// sum 1).  The image and kernel estimates stay in their caller layouts,
+// each engine's estimate the other's blurring input.  initial (host,
+// image dims) or NULL = the observation.  *done = false when the shapes are
 // not the fused engines' (different or odd dimensions, unsupported sizes).
 ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kernel,
-                               float *out, float *kernel_out, int outer, int inner,
-                               hipStream_t s, bool *done) {
+                               const float *initial, float *out, float *kernel_out,
+                               int outer, int inner, hipStream_t s, bool *done) {
   *done = false;
   if (std::getenv("CLARITY_FORCE_GENERIC")) return CLARITY_SUCCESS;
   std::unique_ptr<RLEngine> img = make_fast_rl(w);
@@ -280,7 +287,10 @@ ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kern
   CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
   CL_HIP(hipMemcpyAsync(obs.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
   CL_HIP(hipMemcpyAsync(h.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(o.p, obs.p, sizeof(float) * w.IV, hipMemcpyDeviceToDevice, s));
+  if (initial)
+    CL_HIP(hipMemcpyAsync(o.p, initial, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+  else
+    CL_HIP(hipMemcpyAsync(o.p, obs.p, sizeof(float) * w.IV, hipMemcpyDeviceToDevice, s));
   CL_TRY(k_normalize_fp64(h.f(), w.KV, parts.d(), kParts, s));
   for (int it = 0; it < outer; it++) {
     CL_TRY(img->set_inputs(obs.f(), h.f(), s));
@@ -299,16 +309,19 @@ ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kern
   return CLARITY_SUCCESS;
 }
 
-// Alternating blind RL (see ClarityEx.h).
+// Alternating blind RL (see ClarityEx.h); initial (host, image dims) or
+// NULL = the observation.
 ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
-                         Clarity_Dim3 kd, float *out, float *kernel_out,
-                         int outer, int inner) {
+                         Clarity_Dim3 kd, const float *initial, float *out,
+                         float *kernel_out, int outer, int inner) {
   if (outer < 0 || inner < 0) return CLARITY_INVALID_ARGUMENT;
   Work w;
   CL_TRY(work_geometry(w, in, id, kernel, kd, out));
   hipStream_t s = rt().stream;
   bool fused = false;
-  CL_TRY(blind_rl_fused(w, in, kernel, out, kernel_out, outer, inner, s, &fused));
+  CL_TRY(blind_rl_fused(w, in, kernel, initial, out, kernel_out, outer, inner, s,
+                        &fused));
+  rt().last_engine = fused ? "fast" : "generic";
   if (fused) return CLARITY_SUCCESS;
   CL_TRY(work_upload(w, in, kernel, s));
   DevBuf hft, oft, spec, o, s1, s2, parts, scal;
@@ -322,7 +335,16 @@ ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
   CL_TRY(scal.alloc(sizeof(float) * 4));  // E, sum, scale
   float *E = scal.f(), *S = scal.f() + 1, *c = scal.f() + 2;
   float *h = w.psf.f();
-  CL_HIP(hipMemcpyAsync(o.p, w.obs.p, sizeof(float) * w.V, hipMemcpyDeviceToDevice, s));
+  if (initial) {
+    DevBuf init;
+    CL_TRY(init.alloc(sizeof(float) * w.IV));
+    CL_HIP(hipMemcpyAsync(init.p, initial, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+    const int zero[3] = {0, 0, 0};
+    CL_TRY(k_pad_shift(o.f(), w.wd, init.f(), w.id, zero, 0.f, s));
+    CL_HIP(hipStreamSynchronize(s));
+  } else {
+    CL_HIP(hipMemcpyAsync(o.p, w.obs.p, sizeof(float) * w.V, hipMemcpyDeviceToDevice, s));
+  }
   CL_TRY(k_normalize_fp64(h, w.V, parts.d(), kParts, s));
   CL_TRY(k_sum_partials(w.obs.f(), w.V, parts.d(), kParts, s));
   CL_TRY(k_finish_sum(parts.d(), kParts, E, nullptr, nullptr, s));
@@ -340,7 +362,8 @@ ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
     for (int k = 0; k < inner; k++) {
       CL_TRY(convolve_otf(w, h, oft.c(), s1.f(), spec.c(), s));
       CL_TRY(k_divide(s1.f(), w.obs.f(), s1.f(), 0.f, w.V, s));
-      CL_TRY(convolve_otf(w, s1.f(), oft.c(), s2.f(), spec.c(), s));
+      // the correction correlates the ratio with the image estimate
+      CL_TRY(convolve_otf(w, s1.f(), oft.c(), s2.f(), spec.c(), s, true));
       CL_TRY(k_multiply_sum(h, h, s2.f(), w.V, parts.d(), kParts, s));
       CL_TRY(k_normalize_fp64(h, w.V, parts.d(), kParts, s));
     }
@@ -449,6 +472,25 @@ ClarityResult_t idiv_api(const float *g, const float *ghat, int64_t n,
   return CLARITY_SUCCESS;
 }
 
+// reference Clarity_DivideArraysComponentWise (src/ComputePrimitives.cxx:
+// 121-158) computed by the fused passes' guarded ratio, on host arrays
+ClarityResult_t divide_api(float *result, const float *a, const float *b,
+                           float value, int64_t n) {
+  if (!result || !a || !b || n <= 0) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  hipStream_t s = rt().stream;
+  DevBuf da, db, dr;
+  CL_TRY(da.alloc(sizeof(float) * n));
+  CL_TRY(db.alloc(sizeof(float) * n));
+  CL_TRY(dr.alloc(sizeof(float) * n));
+  CL_HIP(hipMemcpyAsync(da.p, a, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_HIP(hipMemcpyAsync(db.p, b, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_TRY(k_guarded_divide(dr.f(), da.f(), db.f(), value, n, s));
+  CL_HIP(hipMemcpyAsync(result, dr.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  CL_HIP(hipStreamSynchronize(s));
+  return CLARITY_SUCCESS;
+}
+
 // Zero-filling stubs kept for ABI completeness (reference
 // src/SmoothedJansenVanCittertDeconvolve.cxx:28-39,
 // src/IDivergenceDeconvolve.cxx:59-72).
@@ -502,8 +544,17 @@ ClarityResult_t ClarityEx_BlindRL(float *inImage, Clarity_Dim3 imageDim,
                                   float *outImage, float *kernelOut, int outer,
                                   int inner) {
   API_LOCK;
-  return blind_rl(inImage, imageDim, kernelImage, kernelDim, outImage,
+  return blind_rl(inImage, imageDim, kernelImage, kernelDim, nullptr, outImage,
                   kernelOut, outer, inner);
+}
+
+ClarityResult_t ClarityEx_BlindRLFrom(float *inImage, Clarity_Dim3 imageDim,
+                                      float *kernelImage, Clarity_Dim3 kernelDim,
+                                      float *initialImage, float *outImage,
+                                      float *kernelOut, int outer, int inner) {
+  API_LOCK;
+  return blind_rl(inImage, imageDim, kernelImage, kernelDim, initialImage,
+                  outImage, kernelOut, outer, inner);
 }
 
 ClarityResult_t Clarity_SmoothedJansenVanCittertDeconvolve(
@@ -553,6 +604,13 @@ ClarityResult_t ClarityEx_IDivergence(const float *g, const float *ghat,
                                       int64_t n, double *result) {
   API_LOCK;
   return idiv_api(g, ghat, n, result);
+}
+
+ClarityResult_t ClarityEx_DivideArraysComponentWise(float *result, const float *a,
+                                                    const float *b, float value,
+                                                    int64_t n) {
+  API_LOCK;
+  return divide_api(result, a, b, value, n);
 }
 
 }  // extern "C"

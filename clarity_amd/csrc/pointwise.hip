@@ -23,12 +23,15 @@ unsigned blocks_for(long long n, int per_thread = 4) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
        i += (long long)gridDim.x * blockDim.x)
 
+// conj_b: the product with conj(b) = (c, -d) in the same operation order
+// (a correlation instead of a convolution; the negation is exact)
 __global__ void modulate_kernel(const float2 *__restrict__ a,
                                 const float2 *__restrict__ b, float2 *out,
-                                long long n, float scale) {
+                                long long n, float scale, int conj_b) {
 #pragma clang fp contract(off)
   GRID_STRIDE(i, n) {
     float2 x = a[i], y = b[i];
+    if (conj_b) y.y = -y.y;
     float re = x.x * y.x - x.y * y.y;
     float im = x.x * y.y + x.y * y.x;
     out[i] = make_float2(scale * re, scale * im);
@@ -63,6 +66,12 @@ __global__ void divide_kernel(float *r, const float *__restrict__ a,
     float d = b[i];
     r[i] = ((double)fabsf(d) < 1e-5) ? value : a[i] / d;
   }
+}
+
+__global__ void guarded_divide_kernel(float *r, const float *__restrict__ a,
+                                      const float *__restrict__ b, float value,
+                                      long long n) {
+  GRID_STRIDE(i, n) r[i] = guarded_ratio(a[i], b[i], value);
 }
 
 template <typename T>
@@ -238,9 +247,9 @@ __global__ void fill_kernel(float *dst, long long n, float v) {
 }  // namespace
 
 ClarityResult_t k_modulate(const float2 *a, const float2 *b, float2 *out,
-                           long long n, float scale, hipStream_t s) {
+                           long long n, float scale, hipStream_t s, bool conj_b) {
   hipLaunchKernelGGL(modulate_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s,
-                     a, b, out, n, scale);
+                     a, b, out, n, scale, conj_b ? 1 : 0);
   return launch_check();
 }
 
@@ -248,6 +257,13 @@ ClarityResult_t k_wiener(const float2 *in, const float2 *psf, float2 *out,
                          long long n, float scale, float eps, hipStream_t s) {
   hipLaunchKernelGGL(wiener_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                      in, psf, out, n, scale, eps);
+  return launch_check();
+}
+
+ClarityResult_t k_guarded_divide(float *r, const float *a, const float *b,
+                                 float value, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(guarded_divide_kernel, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                     r, a, b, value, n);
   return launch_check();
 }
 

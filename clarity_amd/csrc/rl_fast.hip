@@ -525,7 +525,7 @@ struct VolTab {
   Geom g;
   Bufs b;
   int blk0;    // the volume's first block in the launch
-  int flip;    // z_mod: product times (-1)^(kx + ky + kz)
+  int conj;    // z_mod: product with the kernel column's conjugate
   int nparts;  // finish: partial sums of the volume's last x pass
 };
 
@@ -780,18 +780,6 @@ __device__ __forceinline__ void store_row_pair(float *row, int n0, int ix, float
   }
 }
 
-// a / b for normal b (the guarded ratio has |b| >= 1e-5): the hardware
-// reciprocal (1 ulp) and one Newton correction of the quotient, q + (a - b
-// q) / b with the residual exact in one fma; within about 1 ulp of the
-// correctly rounded quotient in 4 instructions, where IEEE division takes
-// about 10 (scaling, reciprocal, refinement, fix-up for b near the range
-// limits, which the guard excludes).
-__device__ __forceinline__ float quotient(float a, float b) {
-  const float r = __builtin_amdgcn_rcpf(b);
-  const float q = a * r;
-  return fmaf(fmaf(-b, q, a), r, q);
-}
-
 // x kernels share one shared-memory block: the TP-row tile, then the M-point
 // and (M+1)-entry twiddle tables.
 // Tile index of this workgroup.  With 16-row tiles a tile spans 128 bytes
@@ -909,11 +897,11 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
     float r0 = 0.f, r1 = 0.f;
     if (valid && in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].x;
-      r0 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].x, s);
+      r0 = guarded_ratio(o[e].x, s, 0.f);
     }
     if (valid && second_in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].y;
-      r1 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].y, s);
+      r1 = guarded_ratio(o[e].y, s, 0.f);
     }
     v[e] = make_float2(r0, r1);
   }
@@ -1256,7 +1244,7 @@ __global__ void __launch_bounds__(PPW *(N / E))
 // tests.
 template <int N, int E, int TP, int HZ, bool WIENER = false>
 __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, int flip, float nn2 = 0.f, float eps = 0.f) {
+        const float2 *__restrict__ gtw, Geom g, int conjk, float nn2 = 0.f, float eps = 0.f) {
   constexpr bool H = HZ > 0;
   static_assert(HZ % 2 == 0 && HZ <= E, "kernel planes in whole elements");
   constexpr int T = N / E;
@@ -1330,6 +1318,12 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
                 KO = HALF ? kKeepLower : 0;
   fft_pruned<N, E, 1, -1, ZV, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
   fft_pruned<N, E, 1, -1, ZH, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
+  if (conjk) {
+    // a correlation: the product with conj of the kernel column's 3D
+    // spectrum (the negation is exact)
+#pragma unroll
+    for (int e = 0; e < E; e++) h[e].y = -h[e].y;
+  }
   if constexpr (WIENER) {
     // reference Wiener filter (src/WienerDeconvolve.cxx:54-64): Hc =
     // conj(H) (1 / (|H|^2 + eps)), then Hc G s.  K holds s H (s = 1/n, n
@@ -1345,17 +1339,6 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
     // folded into K
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
-  }
-  if (flip) {
-    // the product also times (-1)^(kx + ky + kz), a cyclic shift by half
-    // the work domain on every axis.  After the forward transforms element
-    // e holds z frequency t + T*e, which has t's parity (T is even): one
-    // sign for the lane's whole column
-    const unsigned m = ((kx + y0 + q + t) & 1) ? 0x80000000u : 0u;
-#pragma unroll
-    for (int e = 0; e < E; e++)
-      v[e] = make_float2(__uint_as_float(__float_as_uint(v[e].x) ^ m),
-                         __uint_as_float(__float_as_uint(v[e].y) ^ m));
   }
   fft_pruned<N, E, 1, +1, 0, KO>(v, t, L, tw, typename Plan<N, E>::Rad{});
   {
@@ -1374,14 +1357,14 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
 template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, int flip) {
-  z_body<N, E, TP, HZ>(blockIdx.x, S, K, gtw, g, flip);
+        const float2 *__restrict__ gtw, Geom g, int conjk) {
+  z_body<N, E, TP, HZ>(blockIdx.x, S, K, gtw, g, conjk);
 }
 template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
-  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.flip);
+  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.conj);
 }
 // z pass of the Wiener filter (single volume)
 template <int N, int E, int TP, int HZ>
@@ -1693,9 +1676,9 @@ void dispatch_y(const Tune &tu, bool fwd, const Geom &g, const Bufs &b, hipStrea
 }
 
 // What z_mod does with the spectrum column: the product with the OTF,
-// with the kernel step's sign (flip), or the Wiener filter.
+// with its conjugate (the kernel step's correlation), or the Wiener filter.
 struct ZOp {
-  bool flip = false;
+  bool conj = false;
   bool wiener = false;
   float nn2 = 0.f, eps = 0.f;  // Wiener: (nx ny nz)^2 and epsilon
 };
@@ -1709,7 +1692,7 @@ void launch_z_hz(const ZOp &op, const Geom &g, const Bufs &b, unsigned grid, hip
   }
   if (recorded(&kb_z<N, E, TP, HZ>, grid, TP * (N / E))) return;
   hipLaunchKernelGGL((k_z<N, E, TP, HZ>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.K,
-                     b.twZ, g, (int)op.flip);
+                     b.twZ, g, (int)op.conj);
 }
 
 template <int N, int E, int TP>
@@ -1721,7 +1704,7 @@ void launch_z_t(int zw, const ZOp &op, const Geom &g, const Bufs &b, hipStream_t
   const unsigned long long extent = (unsigned long long)(N / 2) * g.sz * sizeof(float2);
   const bool fits32 = 2 * extent < (1ull << 32);
   const bool half = 2 * g.iz == N && 2 * g.kz == N;
-  if (half && zw > 1 && fits32 && !op.flip && !op.wiener && !g_rec) {
+  if (half && zw > 1 && fits32 && !op.conj && !op.wiener && !g_rec) {
     hipLaunchKernelGGL((k_zp<N, E, TP>), dim3((unsigned)((ntot + zw - 1) / zw)),
                        dim3(TP * (N / E)), 0, s, b.S, b.K, b.twZ, g);
     return;
@@ -1889,7 +1872,7 @@ class FastRL : public RLEngine {
       case KY_INV: dispatch_y(tu_, false, g_, b_, nullptr); break;
       case KZ_MOD: {
         ZOp op;
-        op.flip = flip_ && second;
+        op.conj = corr_ && second;
         dispatch_z(tu_, g_, b_, nullptr, op);
         break;
       }
@@ -1899,13 +1882,13 @@ class FastRL : public RLEngine {
     return r;
   }
   VolTab table_entry(bool second) const {
-    return VolTab{g_, b_, 0, (flip_ && second) ? 1 : 0, nparts_};
+    return VolTab{g_, b_, 0, (corr_ && second) ? 1 : 0, nparts_};
   }
 
-  // The kernel half-step of alternating blind RL: the correction
-  // convolution's spectrum times (-1)^(kx + ky + kz), the estimate
+  // The kernel half-step of alternating blind RL: the correction a
+  // correlation (the blurring input's conjugate spectrum), the estimate
   // renormalised to sum 1 (see make_fast_rl_kernel_step).
-  void make_kernel_step() { flip_ = unit_ = true; }
+  void make_kernel_step() { corr_ = unit_ = true; }
 
   // Jansen-van Cittert instead of RL (make_fast_jvc): each iteration one
   // convolution and u' = max(0, u + gamma (obs - H (*) u)); A and 1/A^2 are
@@ -2076,9 +2059,9 @@ class FastRL : public RLEngine {
     }
     for (int half = 0; half < 2; half++) {
       CL_TRY(run(KY_FWD, s));
-      flip_now_ = flip_ && half == 1;
+      corr_now_ = corr_ && half == 1;
       const ClarityResult_t r = run(KZ_MOD, s);
-      flip_now_ = false;
+      corr_now_ = false;
       CL_TRY(r);
       CL_TRY(run(KY_INV, s));
       CL_TRY(run(half == 0 ? KX_MID : chained ? KX_LAST_FWD : KX_LAST, s));
@@ -2304,7 +2287,7 @@ class FastRL : public RLEngine {
       case KY_INV: dispatch_y(tu_, false, g_, b_, s); break;
       case KZ_MOD: {
         ZOp op;
-        op.flip = flip_now_;
+        op.conj = corr_now_;
         op.wiener = wiener_;
         op.nn2 = nn2_;
         op.eps = eps_;
@@ -2333,9 +2316,9 @@ class FastRL : public RLEngine {
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, K_, parts_, scal_, pad_;
-  bool flip_ = false;      // kernel step: the correction's spectrum sign
+  bool corr_ = false;      // kernel step: the correction correlates
   bool unit_ = false;      // kernel step: renormalise to sum 1
-  bool flip_now_ = false;  // the z_mod being enqueued takes the sign
+  bool corr_now_ = false;  // the z_mod being enqueued takes the conjugate
   bool jvc_ = false;       // Jansen-van Cittert iterations
   bool norescale_ = false; // RL without the energy rescale
   bool out_ = false;       // one convolution / Wiener filter
@@ -2572,20 +2555,19 @@ std::unique_ptr<RLEngine> make_fast_rl(const Work &ref) {
   return std::unique_ptr<RLEngine>(new FastRL(geom));
 }
 
-// The kernel half-step of alternating blind RL, h' = h * (O (*) guard(obs /
-// (O (*) h))) then h renormalised to sum 1, as an RL iteration of this
-// engine: the estimate is the kernel in its caller layout (kernelDim, in
-// the corner), the blurring "kernel" input is the image estimate O
-// (imageDim), both convolutions with O and no conjugate, as the generic
-// driver.  Holding h in the corner instead of shifted by -k/2 makes the
-// first convolution the engine's own (O shifted by -i/2); the second needs
-// O shifted by +i/2 instead, which differs by a shift of i.  On the
-// reference work domain with i = k, i.e. n = 2i on every axis, that shift
-// is half the domain: the spectrum times (-1)^(kx + ky + kz), which z_mod
-// applies to the correction.  The reference domain is kept (no smaller
-// alias-free domain): the generic driver's cyclic convolutions wrap around
-// in the correction, and this reproduces them.  nullptr unless image and
-// kernel have the same even dimensions and the domain is supported.
+// The kernel half-step of alternating blind RL, h'(z) = h(z) sum_y r(y)
+// O(y - z) with r = guard(obs / (h (*) O)), then h renormalised to sum 1,
+// as an RL iteration of this engine: the estimate is the kernel in its
+// caller layout (kernelDim, in the corner), the blurring "kernel" input is
+// the image estimate O (imageDim).  Holding h in the corner instead of
+// shifted by -k/2 makes the first convolution the engine's own (O shifted
+// by -i/2, the same shift when i = k); the correction is the correlation
+// of r with that shifted O, i.e. the second product takes conj of its
+// spectrum, which z_mod applies.  On the reference work domain (n = i + k
+// = 2i on every axis) the correlation of r (zero outside the image corner)
+// with O does not wrap around, so the result is the generic driver's.
+// nullptr unless image and kernel have the same even dimensions and the
+// domain is supported.
 std::unique_ptr<RLEngine> make_fast_rl_kernel_step(const Work &ref) {
   const Dims &i = ref.id, &k = ref.kd;
   if (i.x != k.x || i.y != k.y || i.z != k.z) return nullptr;

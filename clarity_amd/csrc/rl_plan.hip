@@ -96,6 +96,7 @@ std::unique_ptr<RLEngine> make_generic_rl(const Work &geom) {
 using namespace clarity;
 
 struct ClarityEx_RLPlan_t {
+  RegHold reg;  // first: released after the engine's buffers
   Work geom;
   std::unique_ptr<RLEngine> eng;
   bool have_inputs = false;
@@ -110,6 +111,7 @@ ClarityResult_t plan_create(ClarityEx_RLPlan *plan, Clarity_Dim3 id,
   float dummy = 0.f;
   std::unique_ptr<ClarityEx_RLPlan_t> p(new ClarityEx_RLPlan_t);
   CL_TRY(work_geometry(p->geom, &dummy, id, &dummy, kd, &dummy));
+  CL_TRY(p->reg.acquire());
   p->eng = make_fast_rl(p->geom);
   if (!p->eng || std::getenv("CLARITY_FORCE_GENERIC"))
     p->eng = make_generic_rl(p->geom);
@@ -176,6 +178,7 @@ ClarityResult_t ml_call(float *in, Clarity_Dim3 id, float *kernel,
   ClarityEx_RLPlan p = nullptr;
   CL_TRY(plan_create(&p, id, kd));
   std::unique_ptr<ClarityEx_RLPlan_t> hold(p);
+  rt().last_engine = p->eng->name();
   CL_TRY(plan_set_inputs(p, in, kernel, 0));
   if (init) CL_TRY(plan_reset(p, init, 0));
   CL_TRY(p->eng->iterate(iterations, rt().stream));

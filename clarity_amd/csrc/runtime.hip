@@ -195,6 +195,8 @@ ClarityResult_t ClarityEx_Synchronize() {
 
 const char *ClarityEx_DeviceName() { return rt().device_name.c_str(); }
 
+const char *ClarityEx_LastEngine() { return rt().last_engine; }
+
 ClarityResult_t ClarityEx_InjectDeviceOOM(int64_t every) {
   Runtime &r = rt();
   std::lock_guard<std::recursive_mutex> lk(r.mu);

@@ -1021,6 +1021,7 @@ class UNet {
 using namespace clarity;
 
 struct ClarityEx_UNet_t {
+  RegHold reg;  // first: released after the network's buffers
   UNet net;
 };
 
@@ -1080,6 +1081,7 @@ ClarityResult_t ClarityEx_UNetCreate(ClarityEx_UNet *net, unsigned seed) {
   *net = nullptr;
   if (!ready()) return CLARITY_INVALID_OPERATION;
   std::unique_ptr<ClarityEx_UNet_t> h(new ClarityEx_UNet_t);
+  CL_TRY(h->reg.acquire());
   CL_TRY(h->net.init(seed));
   *net = h.release();
   return CLARITY_SUCCESS;

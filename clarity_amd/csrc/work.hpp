@@ -33,8 +33,10 @@ ClarityResult_t work_pad_device(Work &w, const float *in_d,
                                 const float *kernel_d, hipStream_t s);
 
 // out = C2R(modulate(R2C(in), otf)) over the work domain; spec is scratch.
+// conj: modulate with conj(otf), i.e. the correlation with otf's volume.
 ClarityResult_t convolve_otf(const Work &w, const float *in, const float2 *otf,
-                             float *out, float2 *spec, hipStream_t s);
+                             float *out, float2 *spec, hipStream_t s,
+                             bool conj = false);
 
 // Clips the low corner of a work-domain buffer and copies it to the host.
 ClarityResult_t work_download(const Work &w, const float *src, float *host_out,

@@ -25,11 +25,15 @@ C_FUNC_DEF ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveFrom(
 
 /* Alternating blind Richardson-Lucy.  Completes the kernel step that the
  * reference leaves commented out (src/BlindMaximumLikelihoodDeconvolve.cxx:
- * 89-96), with the reference's conventions (no conjugate, guard -> 0).  Per
- * outer iteration: `inner` image RL updates with energy rescale, then
- * `inner` kernel updates h' = h * (O (*) guard(obs / (O (*) h))), each
- * followed by renormalising h to sum 1 in float64.  kernelOut (kernelDim
- * floats, may be NULL) receives the final kernel. */
+ * 89-96).  Per outer iteration: `inner` image RL updates exactly as
+ * Clarity_MaximumLikelihoodDeconvolve (H in both convolutions, guard -> 0,
+ * energy rescale), then `inner` kernel updates
+ *   h'(z) = h(z) * sum_y r(y) O(y - z),   r = guard(obs / (h (*) O)),
+ * the Richardson-Lucy step for h with the image O fixed (the correction is
+ * the correlation of r with O, spectrum R conj(O); the commented-out
+ * reference line modulates with O itself, a convolution, which moves even
+ * the true kernel), each followed by renormalising h to sum 1 in float64.
+ * kernelOut (kernelDim floats, may be NULL) receives the final kernel. */
 C_FUNC_DEF ClarityResult_t ClarityEx_BlindRL(float *inImage,
                                              Clarity_Dim3 imageDim,
                                              float *kernelImage,
@@ -37,6 +41,17 @@ C_FUNC_DEF ClarityResult_t ClarityEx_BlindRL(float *inImage,
                                              float *outImage,
                                              float *kernelOut, int outer,
                                              int inner);
+
+/* ClarityEx_BlindRL from a caller-given first image estimate (imageDim
+ * floats; NULL = the observation, ClarityEx_BlindRL's start). */
+C_FUNC_DEF ClarityResult_t ClarityEx_BlindRLFrom(float *inImage,
+                                                 Clarity_Dim3 imageDim,
+                                                 float *kernelImage,
+                                                 Clarity_Dim3 kernelDim,
+                                                 float *initialImage,
+                                                 float *outImage,
+                                                 float *kernelOut, int outer,
+                                                 int inner);
 
 /* I-divergence sum_i g ln(g/ghat) + ghat - g over n host floats, float64 on
  * the device, 0 ln 0 := 0 (reference functional src/IDivergenceDeconvolve.cxx:
@@ -60,7 +75,9 @@ C_FUNC_DEF ClarityResult_t ClarityEx_MaximumLikelihoodDeconvolveBatch(
 typedef struct ClarityEx_RLPlan_t *ClarityEx_RLPlan;
 
 /* Creates a plan for one (imageDim, kernelDim) pair: allocates the HBM
- * buffers and FFT twiddles.  No data yet. */
+ * buffers and FFT twiddles.  No data yet.  The plan holds a registration
+ * (Clarity_Register) until ClarityEx_RLPlanDestroy, so a caller's last
+ * Clarity_UnRegister does not free what the plan uses. */
 C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanCreate(ClarityEx_RLPlan *plan,
                                                   Clarity_Dim3 imageDim,
                                                   Clarity_Dim3 kernelDim);
@@ -168,6 +185,22 @@ C_FUNC_DEF ClarityResult_t ClarityEx_UNetTrainStep(
     ClarityEx_UNet net, const float *obs, const float *target, Clarity_Dim3 dims,
     float sigmaXY, float sigmaZ, float learningRate, float *loss, float *grads,
     int onDevice);
+
+/* Clarity_DivideArraysComponentWise (reference include/internal/
+ * ComputePrimitives.h, src/ComputePrimitives.cxx:121-158) over n host floats,
+ * computed as the fused Richardson-Lucy passes compute their ratio:
+ * result = value where |b| < 1e-5 (the reference's double compare, exactly),
+ * else a / b by the hardware reciprocal and one Newton step, within 1 ulp of
+ * IEEE division for 1e-5 <= |b| < 2^126. */
+C_FUNC_DEF ClarityResult_t ClarityEx_DivideArraysComponentWise(float *result,
+                                                               const float *a,
+                                                               const float *b,
+                                                               float value,
+                                                               int64_t n);
+
+/* Engine of the last Convolve, Wiener, Jansen-van Cittert, (blind) ML or
+ * blind RL call: "fast" (the fused passes) or "generic" ("" before any). */
+C_FUNC_DEF const char *ClarityEx_LastEngine();
 
 /* Blocks until all work queued on the library stream has finished. */
 C_FUNC_DEF ClarityResult_t ClarityEx_Synchronize();

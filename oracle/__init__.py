@@ -67,8 +67,8 @@ def lib():
         L.oracle_blind_ml.argtypes = drv + [ci]
         L.oracle_ml_ex.argtypes = [f32p, ci, ci, ci, f32p, ci, ci, ci,
                                    ctypes.c_void_p, f32p, ci, ctypes.c_void_p]
-        L.oracle_blind_rl.argtypes = [f32p, ci, ci, ci, f32p, ci, ci, ci, f32p,
-                                      ctypes.c_void_p, ci, ci]
+        L.oracle_blind_rl.argtypes = [f32p, ci, ci, ci, f32p, ci, ci, ci,
+                                      ctypes.c_void_p, f32p, ctypes.c_void_p, ci, ci]
         L.oracle_idivergence.argtypes = [f32p, f32p, i64]
         L.oracle_idivergence.restype = ctypes.c_double
         L.oracle_fft_backend_name.restype = ctypes.c_char_p
@@ -282,14 +282,19 @@ def ml_ex(img, psf, iterations, initial=None, want_trace=False):
     return (out, trace[:iterations]) if want_trace else out
 
 
-def blind_rl(img, psf, outer, inner):
+def blind_rl(img, psf, outer, inner, initial=None):
+    """Alternating blind RL; returns (image, kernel).  ``initial`` is the
+    first image estimate (None: the observation)."""
     img = _f32(img)
     psf = _f32(psf)
     ix, iy, iz = _dims(img)
     kx, ky, kz = _dims(psf)
     out = np.empty_like(img)
     psf_out = np.empty_like(psf)
-    rc = lib().oracle_blind_rl(img, ix, iy, iz, psf, kx, ky, kz, out,
+    init = None if initial is None else _f32(initial)
+    assert init is None or init.shape == img.shape
+    rc = lib().oracle_blind_rl(img, ix, iy, iz, psf, kx, ky, kz,
+                               None if init is None else init.ctypes.data, out,
                                psf_out.ctypes.data, outer, inner)
     _check(rc, "blind_rl")
     return out, psf_out

@@ -188,16 +188,33 @@ static void fft_pencil(const fplan *pl, double *x, double *y, int sign) {
           ur[r] = a * c - b * d;
           ui[r] = a * d + b * c;
         }
-        for (int q = 0; q < R; q++) {
-          double sr = 0.0, si = 0.0;
-          for (int r = 0; r < R; r++) {
-            int ti = (int)(((long)r * q) % R) * m;
-            double c = pl->wr[ti], d = sign < 0 ? pl->wi[ti] : -pl->wi[ti];
-            sr += ur[r] * c - ui[r] * d;
-            si += ur[r] * d + ui[r] * c;
+        if (R == 2) {
+          vr[0] = ur[0] + ur[1]; vi[0] = ui[0] + ui[1];
+          vr[1] = ur[0] - ur[1]; vi[1] = ui[0] - ui[1];
+        } else if (R == 4) {
+          /* X_q = sum_r u_r w^(rq), w = exp(sign 2 pi i / 4) = sign * i */
+          const double s = (double)sign;
+          const double ar = ur[0] + ur[2], ai = ui[0] + ui[2];
+          const double br = ur[0] - ur[2], bi = ui[0] - ui[2];
+          const double cr = ur[1] + ur[3], ci = ui[1] + ui[3];
+          const double dr = ur[1] - ur[3], di = ui[1] - ui[3];
+          vr[0] = ar + cr; vi[0] = ai + ci;
+          vr[2] = ar - cr; vi[2] = ai - ci;
+          /* (sign i) d = (-s di, s dr) */
+          vr[1] = br - s * di; vi[1] = bi + s * dr;
+          vr[3] = br + s * di; vi[3] = bi - s * dr;
+        } else {
+          for (int q = 0; q < R; q++) {
+            double sr = 0.0, si = 0.0;
+            for (int r = 0; r < R; r++) {
+              int ti = (int)(((long)r * q) % R) * m;
+              double c = pl->wr[ti], d = sign < 0 ? pl->wi[ti] : -pl->wi[ti];
+              sr += ur[r] * c - ui[r] * d;
+              si += ur[r] * d + ui[r] * c;
+            }
+            vr[q] = sr;
+            vi[q] = si;
           }
-          vr[q] = sr;
-          vi[q] = si;
         }
         const int base = (i / p) * p * R + k;
         for (int r = 0; r < R; r++) {
@@ -240,28 +257,39 @@ static void fft_pencil(const fplan *pl, double *x, double *y, int sign) {
 
 /* Complex transform along one axis of an interleaved double array.  Pencil
  * (a, b) has element j at ((a * nb) + b) + j * stride, in complex units,
- * where the axis sits between the outer count na and the inner count nb. */
+ * where the axis sits between the outer count na and the inner count nb.
+ * Pencils are gathered AXIS_BLOCK at a time along b (adjacent in memory),
+ * so every cache line read or written is used whole. */
+#define AXIS_BLOCK 8
 static void axis_pass(double *w, int n, int64_t na, int64_t nb, int64_t stride,
                       int64_t astride, int sign) {
   fplan pl;
   fplan_init(&pl, n);
-  const int64_t total = na * nb;
+  const int64_t nblk = (nb + AXIS_BLOCK - 1) / AXIS_BLOCK;
+  const int64_t total = na * nblk;
 #pragma omp parallel
   {
-    double *x = (double *)malloc(sizeof(double) * 4 * n);
-    double *y = x + 2 * n;
+    double *x = (double *)malloc(sizeof(double) * 2 * n * (AXIS_BLOCK + 1));
+    double *y = x + 2 * n * AXIS_BLOCK;
 #pragma omp for schedule(static)
     for (int64_t pidx = 0; pidx < total; pidx++) {
-      int64_t a = pidx / nb, b = pidx % nb;
-      double *base = w + 2 * (a * astride + b);
+      const int64_t a = pidx / nblk, b0 = (pidx % nblk) * AXIS_BLOCK;
+      const int cnt = (int)(nb - b0 < AXIS_BLOCK ? nb - b0 : AXIS_BLOCK);
+      double *base = w + 2 * (a * astride + b0);
       for (int j = 0; j < n; j++) {
-        x[2 * j] = base[2 * (int64_t)j * stride];
-        x[2 * j + 1] = base[2 * (int64_t)j * stride + 1];
+        const double *src = base + 2 * (int64_t)j * stride;
+        for (int c = 0; c < cnt; c++) {
+          x[2 * (c * n + j)] = src[2 * c];
+          x[2 * (c * n + j) + 1] = src[2 * c + 1];
+        }
       }
-      fft_pencil(&pl, x, y, sign);
+      for (int c = 0; c < cnt; c++) fft_pencil(&pl, x + 2 * c * n, y, sign);
       for (int j = 0; j < n; j++) {
-        base[2 * (int64_t)j * stride] = x[2 * j];
-        base[2 * (int64_t)j * stride + 1] = x[2 * j + 1];
+        double *dst = base + 2 * (int64_t)j * stride;
+        for (int c = 0; c < cnt; c++) {
+          dst[2 * c] = x[2 * (c * n + j)];
+          dst[2 * c + 1] = x[2 * (c * n + j) + 1];
+        }
       }
     }
     free(x);
@@ -830,13 +858,20 @@ int oracle_blind_ml(const float *in, int ix, int iy, int iz,
 
 /* Alternating blind RL (no counterpart in the reference; it completes the
  * PSF step the reference leaves commented out at
- * src/BlindMaximumLikelihoodDeconvolve.cxx:89-96, with the reference's own
- * conventions: no conjugate, guard 0).  Per outer iteration: `inner` object
- * RL updates with energy rescale (as ml_update), then `inner` PSF updates
- * h' = h * (O (*) guard(obs / (O (*) h))) each followed by renormalising h
- * to sum 1 in float64.  h starts as the padded, shifted kernel normalised to
- * sum 1 in float64.  psf_out (optional) receives h un-shifted and clipped to
- * the kernel dims. */
+ * src/BlindMaximumLikelihoodDeconvolve.cxx:89-96).  Per outer iteration:
+ * `inner` object RL updates with energy rescale (as ml_update, H in both
+ * convolutions, guard 0), then `inner` PSF updates
+ *   h'(z) = h(z) * sum_y r(y) o(y - z),  r = guard(obs / (h (*) o)),
+ * i.e. the correction is the CORRELATION of r with o (spectrum R conj(O)),
+ * each followed by renormalising h to sum 1 in float64.  This is the
+ * Richardson-Lucy step for h with o fixed (the multiplicative EM update of
+ * the Poisson likelihood in h); the commented-out reference line modulates
+ * with O itself (a convolution), which is that step only for an object
+ * symmetric about the origin: with the true object and PSF it moves the
+ * PSF by O(1), where the correlation leaves it fixed.  h starts as the
+ * padded, shifted kernel normalised to sum 1 in float64; o as `initial`
+ * (padded, image dims) or, NULL, the padded observation.  psf_out
+ * (optional) receives h un-shifted and clipped to the kernel dims. */
 static void normalize_fp64(float *h, int64_t n) {
   double s = 0.0;
 #pragma omp parallel for reduction(+ : s) schedule(static)
@@ -846,9 +881,33 @@ static void normalize_fp64(float *h, int64_t n) {
   for (int64_t i = 0; i < n; i++) h[i] = h[i] * inv;
 }
 
+/* the correction's correlation: C2R(R2C(in) * conj(ft) * scale), the
+ * product in ComplexMultiplyAndScale's order with conj(b) = (c, -d) */
+static int correlate_otf(const work_t *w, const float *in, const float *ft,
+                         float *out) {
+  float *spec = (float *)malloc(sizeof(float) * (size_t)w->c);
+  if (!spec) return OR_OUT_OF_MEMORY;
+  int rc = fft_r2c(w->nx, w->ny, w->nz, in, spec);
+  if (rc == OR_SUCCESS) {
+    const int64_t n = w->c / 2;
+    const float scale = inv_volume(w->nx, w->ny, w->nz);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      float ar = spec[2 * i], ai = spec[2 * i + 1];
+      float br = ft[2 * i], bi = -ft[2 * i + 1];
+      spec[2 * i] = scale * (ar * br - ai * bi);
+      spec[2 * i + 1] = scale * (ar * bi + ai * br);
+    }
+    rc = fft_c2r_destroy(w->nx, w->ny, w->nz, spec, out);
+  }
+  free(spec);
+  return rc;
+}
+
 int oracle_blind_rl(const float *in, int ix, int iy, int iz,
-                    const float *kernel, int kx, int ky, int kz, float *out,
-                    float *psf_out, int outer, int inner) {
+                    const float *kernel, int kx, int ky, int kz,
+                    const float *initial, float *out, float *psf_out,
+                    int outer, int inner) {
   if (!in || !kernel || !out) return OR_INVALID_ARGUMENT;
   if (!dims_ok(ix, iy, iz) || !dims_ok(kx, ky, kz)) return OR_INVALID_ARGUMENT;
   if (outer < 0 || inner < 0) return OR_INVALID_ARGUMENT;
@@ -865,7 +924,12 @@ int oracle_blind_rl(const float *in, int ix, int iy, int iz,
   }
   float *h = w.k_pad;
   if (rc == OR_SUCCESS) {
-    memcpy(o, w.in_pad, sizeof(float) * (size_t)w.v);
+    if (initial) {
+      const int zero[3] = {0, 0, 0};
+      rc = oracle_pad_shift(o, w.nx, w.ny, w.nz, initial, ix, iy, iz, zero, 0.0f);
+    } else {
+      memcpy(o, w.in_pad, sizeof(float) * (size_t)w.v);
+    }
     normalize_fp64(h, w.v);
   }
   float energy = rc == OR_SUCCESS ? oracle_reduce_sum(w.in_pad, w.v) : 0.0f;
@@ -878,7 +942,7 @@ int oracle_blind_rl(const float *in, int ix, int iy, int iz,
       rc = convolve_otf(&w, h, oft, s1);
       if (rc != OR_SUCCESS) break;
       oracle_divide(s1, w.in_pad, s1, 0.0f, w.v);
-      rc = convolve_otf(&w, s1, oft, s2);
+      rc = correlate_otf(&w, s1, oft, s2);
       if (rc != OR_SUCCESS) break;
       oracle_multiply(h, h, s2, w.v);
       normalize_fp64(h, w.v);

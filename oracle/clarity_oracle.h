@@ -103,8 +103,9 @@ int oracle_ml_ex(const float *in, int ix, int iy, int iz, const float *kernel,
                  int kx, int ky, int kz, const float *initial, float *out,
                  int iterations, float *scale_trace);
 int oracle_blind_rl(const float *in, int ix, int iy, int iz,
-                    const float *kernel, int kx, int ky, int kz, float *out,
-                    float *psf_out, int outer, int inner);
+                    const float *kernel, int kx, int ky, int kz,
+                    const float *initial, float *out, float *psf_out,
+                    int outer, int inner);
 double oracle_idivergence(const float *g, const float *ghat, int64_t n);
 
 /* Wall seconds of the iteration loop of the last oracle_ml / oracle_ml_ex

@@ -377,3 +377,38 @@ def test_autotune_leaves_results_unchanged(clarity_gpu):
     again.iterate(4)
     assert rel_l2(again.estimate(), before) <= 1e-5
     again.close()
+
+
+def test_plan_outlives_unregister():
+    """A plan holds a registration: after the caller's last
+    Clarity_UnRegister the plan still iterates (its twiddle tables and the
+    library stream are kept) and gives the result it gives registered;
+    destroying it then tears the runtime down, and the next call registers
+    again.  Run in a fresh process, so that the session's registration does
+    not hide the teardown."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np\n"
+        "from clarity_amd import api, synth\n"
+        "c = synth.make_workload('beads-64x64x32')\n"
+        "api.Register()\n"
+        "ref = api.MaximumLikelihoodDeconvolve(c.obs, c.psf, 6)\n"
+        "p = api.RLPlan(c.obs.shape, c.psf.shape)\n"
+        "p.set_inputs(c.obs, c.psf)\n"
+        "p.iterate(3)\n"
+        "api.UnRegister()\n"
+        "p.iterate(3)\n"
+        "out = p.estimate()\n"
+        "p.close()\n"
+        "e = np.linalg.norm(out.astype(np.float64) - ref) / np.linalg.norm(ref)\n"
+        "print('plan after UnRegister: rel', e, flush=True)\n"
+        "again = api.MaximumLikelihoodDeconvolve(c.obs, c.psf, 6)\n"
+        "print('implicit re-registration: equal', np.array_equal(again, ref), flush=True)\n"
+        "sys.exit(0 if e <= 1e-6 and np.array_equal(again, ref) else 3)\n" % root)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0

@@ -1,0 +1,196 @@
+"""Parity at the benchmark workloads' full sizes.
+
+The fused RL engine at 512x512x128 (100 iterations, same-size side-lobe
+kernel) against the oracle on the FFTW3 float API (the reference's FFT
+library interface; MKL's implementation where libfftw3f is absent) with
+float64 reductions, the library's rule; the deviation from the reference's
+literal sequential float32 energy sum is printed beside it.  At
+256x256x64: RL 100 iterations, Wiener (epsilon 1e-3) and Jansen-van Cittert
+(50 iterations) against the oracle's float64 FFT, so that the error against
+a float64 computation is stated.  Blind RL at 512x512x128 on its 20 x 5
+schedule is checked through properties and the spread between the two
+engines (there is no float64 run of that size here: it would take minutes).
+
+Tolerances (relative L2): RL 100 iterations <= 1e-4, JVC 50 iterations <=
+1e-4 (the targets stated for the float32 path), Wiener <= 1e-6 (measured
+6e-7).
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from clarity_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(autouse=True)
+def _fp64_oracle():
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+    yield
+    oracle.set_reduce_mode(oracle.REDUCE_FP64)
+    oracle.set_fft_backend(oracle.FFT_OWN)
+
+
+_cases = {}
+
+
+def _workload(api, name):
+    """The named workload (seed as the benchmark), observation convolved on
+    the GPU; cached per module."""
+    if name not in _cases:
+        t0 = time.perf_counter()
+        _cases[name] = synth.make_workload(name, convolve=api.Convolve)
+        print(f"[fullsize] {name}: inputs in {time.perf_counter() - t0:.1f} s", flush=True)
+    return _cases[name]
+
+
+_rl512 = {}
+
+
+def _rl512_gpu(api):
+    if "out" not in _rl512:
+        case = _workload(api, "filaments-512x512x128")
+        t0 = time.perf_counter()
+        _rl512["out"] = api.MaximumLikelihoodDeconvolve(case.obs, case.psf, 100)
+        assert api.LastEngine() == "fast"
+        print(f"[fullsize] RL 100 it 512x512x128 on the GPU: "
+              f"{time.perf_counter() - t0:.2f} s (C call, host buffers)", flush=True)
+    return _rl512["out"]
+
+
+def test_rl_512_100_iterations_vs_oracle(clarity_gpu):
+    """The headline workload: 100 fused RL iterations against the oracle
+    (FFTW3 API FFTs, float64 reductions), rel L2 <= 1e-4."""
+    api = clarity_gpu
+    if not oracle.set_fft_backend(oracle.FFT_FFTW_API):
+        pytest.skip("no FFTW3-API library (libfftw3f or MKL) on this host")
+    case = _workload(api, "filaments-512x512x128")
+    got = _rl512_gpu(api)
+    t0 = time.perf_counter()
+    ref = oracle.ml(case.obs, case.psf, 100)
+    err = rel_l2(got, ref)
+    _rl512["fp64_ref"] = ref
+    print(f"[fullsize] oracle ({oracle.fft_backend_name()}, fp64 sums) "
+          f"{time.perf_counter() - t0:.1f} s; rel L2 GPU vs oracle {err:.3e}", flush=True)
+    assert err <= 1e-4
+    assert np.all(got >= 0)
+
+
+def test_rl_512_100_iterations_serial_f32_deviation(clarity_gpu):
+    """The reference's literal arithmetic (sequential float32 energy sum)
+    at 512x512x128: the deviation is reported, and bounded by what the
+    sum's rounding explains -- the outputs agree once both are scaled to the
+    same total (RL is scale-invariant)."""
+    api = clarity_gpu
+    if not oracle.set_fft_backend(oracle.FFT_FFTW_API):
+        pytest.skip("no FFTW3-API library (libfftw3f or MKL) on this host")
+    case = _workload(api, "filaments-512x512x128")
+    got = _rl512_gpu(api)
+    t0 = time.perf_counter()
+    with oracle.reduce_mode(oracle.REDUCE_SERIAL_F32):
+        ref32 = oracle.ml(case.obs, case.psf, 100)
+    err = rel_l2(got, ref32)
+    g = np.asarray(got, np.float64)
+    r = np.asarray(ref32, np.float64)
+    err_scaled = rel_l2(g / g.sum(), r / r.sum())
+    print(f"[fullsize] oracle serial-f32 sums {time.perf_counter() - t0:.1f} s; "
+          f"rel L2 GPU vs serial-f32 oracle {err:.3e}, after scaling to equal "
+          f"sums {err_scaled:.3e}; total ratio {g.sum() / r.sum():.6f}", flush=True)
+    assert err_scaled <= 1e-4
+
+
+def test_rl_256_100_iterations_vs_fp64_fft_oracle(clarity_gpu):
+    """100 fused RL iterations at 256x256x64 against the oracle's float64
+    FFT (the error against a float64 computation)."""
+    api = clarity_gpu
+    case = _workload(api, "filaments-256x256x64")
+    got = api.MaximumLikelihoodDeconvolve(case.obs, case.psf, 100)
+    assert api.LastEngine() == "fast"
+    t0 = time.perf_counter()
+    ref = oracle.ml(case.obs, case.psf, 100)
+    err = rel_l2(got, ref)
+    print(f"[fullsize] RL 100 it 256x256x64: fp64-FFT oracle "
+          f"{time.perf_counter() - t0:.1f} s; rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
+def test_wiener_256_vs_fp64_fft_oracle(clarity_gpu):
+    """Wiener, epsilon 1e-3, on the 256x256x64 filament workload with the
+    side-lobe kernel (fused passes) against the float64-FFT oracle."""
+    api = clarity_gpu
+    case = _workload(api, "filaments-256x256x64")
+    got = api.WienerDeconvolve(case.obs, case.psf, 1e-3)
+    assert api.LastEngine() == "fast"
+    ref = oracle.wiener(case.obs, case.psf, 1e-3)
+    err = rel_l2(got, ref)
+    print(f"[fullsize] Wiener 256x256x64 eps 1e-3: rel L2 vs fp64-FFT oracle {err:.3e}",
+          flush=True)
+    assert err <= 1e-6
+
+
+def test_jvc_256_50_iterations_vs_fp64_fft_oracle(clarity_gpu):
+    """Jansen-van Cittert, 50 iterations, on the 256x256x64 filament
+    workload (fused passes) against the float64-FFT oracle."""
+    api = clarity_gpu
+    case = _workload(api, "filaments-256x256x64")
+    got = api.JansenVanCittertDeconvolve(case.obs, case.psf, 50)
+    assert api.LastEngine() == "fast"
+    t0 = time.perf_counter()
+    ref = oracle.jvc(case.obs, case.psf, 50)
+    err = rel_l2(got, ref)
+    print(f"[fullsize] JVC 50 it 256x256x64: fp64-FFT oracle "
+          f"{time.perf_counter() - t0:.1f} s; rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-4
+
+
+def test_blind_rl_512_20x5(clarity_gpu):
+    """Alternating blind RL on its benchmark schedule, 20 outer x 5 inner
+    iterations at 512x512x128, kernel started as the Gaussian 20 % too
+    wide.  Checks: the kernel stays a distribution (sum 1, non-negative),
+    the image keeps the observation's energy, the first outer iteration
+    moves the kernel toward the true one (later ones drift on toward a
+    narrower kernel, as unregularised blind RL does on noisy data: the
+    printed trajectory); the fused engine against the
+    generic driver (two float32 implementations) gives the schedule's
+    reproducibility spread, bounded by 1e-3."""
+    import os
+    api = clarity_gpu
+    case = _workload(api, "filaments-512x512x128")
+    k0 = synth.blind_initial_kernel("filaments-512x512x128", case.psf.shape)
+    kt = case.psf.astype(np.float64)
+    e0 = rel_l2(k0 / k0.astype(np.float64).sum(), kt)
+    o1, h1 = api.BlindRL(case.obs, k0, 1, 5)
+    t0 = time.perf_counter()
+    o, h = api.BlindRL(case.obs, k0, 20, 5)
+    t_fast = time.perf_counter() - t0
+    assert api.LastEngine() == "fast"
+    os.environ["CLARITY_FORCE_GENERIC"] = "1"
+    try:
+        t0 = time.perf_counter()
+        og, hg = api.BlindRL(case.obs, k0, 20, 5)
+        t_gen = time.perf_counter() - t0
+        assert api.LastEngine() == "generic"
+    finally:
+        os.environ.pop("CLARITY_FORCE_GENERIC", None)
+    e1, e20 = rel_l2(h1, kt), rel_l2(h, kt)
+    so, sh = rel_l2(o, og), rel_l2(h, hg)
+    print(f"[fullsize] blind RL 20x5 512x512x128: fused {t_fast:.2f} s, generic "
+          f"{t_gen:.2f} s; kernel rel L2 to the true kernel: start {e0:.3f}, "
+          f"after 1 outer {e1:.3f}, after 20 {e20:.3f}; fused vs generic: image "
+          f"{so:.2e}, kernel {sh:.2e}", flush=True)
+    assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
+    assert h.min() >= -1e-6 * h.max()
+    assert abs(float(o.astype(np.float64).sum()) / float(case.obs.astype(np.float64).sum())
+               - 1.0) < 1e-4
+    assert e1 < e0
+    assert so <= 1e-3 and sh <= 1e-3

@@ -3,7 +3,7 @@
 Tolerances (relative L2, ||gpu - ref|| / ||ref||), following the stated
 targets for the float32 path:
   FFT vs numpy float64                     <= 2e-6
-  Convolve / Wiener vs oracle              <= 1e-5
+  Convolve / Wiener vs oracle              <= 1e-6 (measured 1e-7..8.5e-7)
   RL after 1 iteration                     <= 1e-5
   RL / JVC / blind after <= 100 iterations <= 1e-4
 Bit-exact: pad / clip.  The oracle runs in its fp64 reduction mode (the
@@ -129,13 +129,17 @@ def test_clip_bit_exact(clarity_gpu):
 
 def _engine(engine, fn):
     """fn() on the fused passes (the default where they apply) or, with
-    engine == "generic", on the generic driver (CLARITY_FORCE_GENERIC)."""
+    engine == "generic", on the generic driver (CLARITY_FORCE_GENERIC); the
+    engine the library reports for the call must be the one asked for."""
+    from clarity_amd import api
     if engine == "generic":
         os.environ["CLARITY_FORCE_GENERIC"] = "1"
     try:
-        return fn()
+        out = fn()
     finally:
         os.environ.pop("CLARITY_FORCE_GENERIC", None)
+    assert api.LastEngine() == ("generic" if engine == "generic" else "fast")
+    return out
 
 
 @pytest.mark.parametrize("engine", ["fused", "generic"])
@@ -145,7 +149,7 @@ def test_convolve_vs_oracle_small(clarity_gpu, small, engine):
     api = clarity_gpu
     got = _engine(engine, lambda: api.Convolve(small.obj, small.psf))
     ref = oracle.convolve(small.obj, small.psf)
-    assert rel_l2(got, ref) <= 1e-5
+    assert rel_l2(got, ref) <= 1e-6
     # the oracle itself against numpy float64 (independent FFT)
     assert rel_l2(ref, synth.np_convolve(small.obj, small.psf)) <= 1e-6
 
@@ -161,7 +165,7 @@ def test_convolve_shapes(clarity_gpu, ishape, kshape):
     k = _rand(kshape, 7)
     got = api.Convolve(a, k)
     ref = synth.np_convolve(a, k)
-    assert rel_l2(got, ref) <= 1e-5
+    assert rel_l2(got, ref) <= 1e-6
 
 
 def test_convolve_delta_identity(clarity_gpu):
@@ -189,7 +193,7 @@ def test_wiener_vs_oracle(clarity_gpu, small, name, engine):
         ref = oracle.wiener(obs, psf, eps)
         err = rel_l2(got, ref)
         print(f"[wiener] {name} {engine} eps {eps}: rel L2 {err:.2e}", flush=True)
-        assert err <= 1e-5, eps
+        assert err <= 1e-6, eps
 
 
 def test_wiener_delta_kernel(clarity_gpu):
@@ -217,9 +221,9 @@ def test_fused_single_and_jvc_odd_shapes(clarity_gpu, ishape, kshape):
     psf = synth.gaussian_kernel(kshape, 1.6, 3.1)
     obs = synth.poisson(synth.np_convolve(obj, psf), rng)
     for name, fn, ref, tol in (
-            ("convolve", lambda: api.Convolve(obj, psf), oracle.convolve(obj, psf), 1e-5),
+            ("convolve", lambda: api.Convolve(obj, psf), oracle.convolve(obj, psf), 1e-6),
             ("wiener", lambda: api.WienerDeconvolve(obs, psf, 1e-2),
-             oracle.wiener(obs, psf, 1e-2), 1e-5),
+             oracle.wiener(obs, psf, 1e-2), 1e-6),
             ("jvc", lambda: api.JansenVanCittertDeconvolve(obs, psf, 4),
              oracle.jvc(obs, psf, 4), 1e-4)):
         got = _engine("fused", fn)
@@ -308,12 +312,7 @@ def test_jvc_vs_oracle(clarity_gpu, small, name, engine):
         case = synth.make_workload("beads-64x64x32", seed=5, shape=(48, 64, 96))
         obs, psf = case.obs, case.psf
     for iters in (0, 1, 5):
-        if engine == "generic":
-            os.environ["CLARITY_FORCE_GENERIC"] = "1"
-        try:
-            got = api.JansenVanCittertDeconvolve(obs, psf, iters)
-        finally:
-            os.environ.pop("CLARITY_FORCE_GENERIC", None)
+        got = _engine(engine, lambda: api.JansenVanCittertDeconvolve(obs, psf, iters))
         ref = oracle.jvc(obs, psf, iters)
         err = rel_l2(got, ref)
         print(f"[jvc] {name} {engine} {iters} it: rel L2 {err:.2e}", flush=True)
@@ -328,12 +327,7 @@ def test_blind_as_written_vs_oracle(clarity_gpu, small, engine):
     api = clarity_gpu
     k0 = synth.blind_initial_kernel("beads-64x64x32",small.psf.shape)
     for iters in (0, 3, 20):
-        if engine == "generic":
-            os.environ["CLARITY_FORCE_GENERIC"] = "1"
-        try:
-            got = api.BlindMaximumLikelihoodDeconvolve(small.obs, k0, iters)
-        finally:
-            os.environ.pop("CLARITY_FORCE_GENERIC", None)
+        got = _engine(engine, lambda: api.BlindMaximumLikelihoodDeconvolve(small.obs, k0, iters))
         ref = oracle.blind_ml(small.obs, k0, iters)
         err = rel_l2(got, ref)
         print(f"[blind ml] {engine} {iters} it: rel L2 {err:.2e}", flush=True)
@@ -349,62 +343,111 @@ def _blind_case(name):
     return case.obs, synth.blind_initial_kernel("beads-64x64x32", shape)
 
 
-# float32-FFT spread of the oracle, (image, kernel) relative L2 between its
-# float64 FFT and MKL's FFTW3 API, measured on the CPU; used where the API
-# is absent, else measured in the test
-_BLIND_SPREAD = {
-    ("beads64", (1, 2)): (1.6e-7, 6.0e-4), ("beads64", (2, 1)): (2.3e-4, 7.4e-4),
-    ("beads64", (2, 2)): (1.05e-3, 6.9e-3), ("96x64x48", (1, 2)): (1.4e-7, 3.5e-4),
-    ("96x64x48", (2, 1)): (4.6e-4, 1.7e-3), ("96x64x48", (2, 2)): (2.3e-3, 1.8e-3),
-}
-_blind_ref = {}
-
-
 @pytest.mark.parametrize("engine", ["fused", "generic"])
-@pytest.mark.parametrize("sched", [(1, 2), (2, 1), (2, 2)])
+@pytest.mark.parametrize("sched", [(1, 2), (2, 1), (2, 2), (3, 2), (5, 5)])
 @pytest.mark.parametrize("name", ["beads64", "96x64x48"])
 def test_blind_rl_vs_oracle(clarity_gpu, name, sched, engine):
     """Alternating blind RL (outer x inner) against the float64-FFT oracle,
     on the fused engines (same-size even shapes: image steps on the RL
-    engine, kernel steps on its kernel-step form) and on the generic driver
-    (CLARITY_FORCE_GENERIC).  The kernel step is ill-conditioned in float32:
-    it keeps the small deviation of O (*) r from its mean, so the oracle's
-    own float32 FFT (the FFTW3 API, as the reference computes) already
-    differs from its float64 FFT by 6e-4 in the kernel after one outer
-    iteration of 2 kernel steps, 7e-3 after 2 x 2, and the two grow
-    chaotically beyond (0.3 at 3 x 2 on 96x64x48).  The bound is a
-    multiple of that spread: 4x, and 8x in the chaotic 2 x 2 regime, where
-    two float32 implementations' distance is a random draw of that scale
-    (two float32 FFTs of one library already differ by 2x from host to
-    host)."""
+    engine, kernel steps on its kernel-step form) and on the generic driver.
+    The oracle's float32 FFT (FFTW3 API) differs from its float64 one by
+    1e-7..1e-6 on these schedules (measured on the CPU), so the float32 bar
+    of 1e-4 applies."""
     api = clarity_gpu
     obs, k0 = _blind_case(name)
-    if engine == "generic":
-        os.environ["CLARITY_FORCE_GENERIC"] = "1"
-    try:
-        o, h = api.BlindRL(obs, k0, *sched)
-    finally:
-        os.environ.pop("CLARITY_FORCE_GENERIC", None)
+    o, h = _engine(engine, lambda: api.BlindRL(obs, k0, *sched))
     key = (name, sched)
     if key not in _blind_ref:
-        ro, rh = oracle.blind_rl(obs, k0, *sched)
-        spread = _BLIND_SPREAD[key]
-        if oracle.set_fft_backend(oracle.FFT_FFTW_API):
-            fo, fh = oracle.blind_rl(obs, k0, *sched)
-            oracle.set_fft_backend(oracle.FFT_OWN)
-            spread = (rel_l2(fo, ro), rel_l2(fh, rh))
-        _blind_ref[key] = (ro, rh, spread)
-    ro, rh, (spread_o, spread_h) = _blind_ref[key]
-    f = 8 if sched == (2, 2) else 4
+        _blind_ref[key] = oracle.blind_rl(obs, k0, *sched)
+    ro, rh = _blind_ref[key]
     eo, eh = rel_l2(o, ro), rel_l2(h, rh)
-    print(f"[blind] {name} {sched[0]}x{sched[1]} {engine}: err o {eo:.2e} h {eh:.2e}; "
-          f"f32-FFT spread o {spread_o:.2e} h {spread_h:.2e}")
-    assert eo <= max(1e-4, f * spread_o)
-    assert eh <= max(1e-4, f * spread_h)
-    assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-4
-    # h * (O (*) r) is non-negative up to the float32 roundoff of a
-    # near-zero convolution value
+    print(f"[blind] {name} {sched[0]}x{sched[1]} {engine}: err o {eo:.2e} h {eh:.2e}")
+    assert eo <= 1e-4 and eh <= 1e-4
+    assert abs(float(h.astype(np.float64).sum()) - 1.0) < 1e-5
     assert h.min() >= -1e-6 * h.max()
+
+
+_blind_ref = {}
+
+
+def _blind_scene():
+    """Noise-free scene with no symmetry about the origin (beads and
+    filaments inside a 16x20x24 volume, a 5x7x7 kernel), so that the blur
+    stays inside the image."""
+    rng = np.random.default_rng(5)
+    obj = np.zeros((16, 20, 24), np.float32)
+    obj[4:12, 4:16, 4:20] = synth.beads_and_filaments((8, 12, 16), 25, 2, rng,
+                                                      fil_len=(6, 14))
+    k_true = synth.gaussian_kernel((5, 7, 7), 1.0, 1.4)
+    obs = synth.np_convolve(obj, k_true).astype(np.float32)
+    return obj, k_true, obs
+
+
+def test_blind_rl_fixed_point_and_recovery(clarity_gpu):
+    """ClarityEx_BlindRLFrom from the true object and kernel on noise-free
+    data keeps both (the kernel step is a correlation); from the
+    observation with the kernel 20 % too wide, the kernel steps move the
+    kernel toward the true one, as the oracle's."""
+    api = clarity_gpu
+    obj, k_true, obs = _blind_scene()
+    kt = k_true / k_true.astype(np.float64).sum()
+    o, h = _engine("generic", lambda: api.BlindRL(obs, k_true, 2, 2, initial=obj))
+    assert rel_l2(h, kt) <= 1e-5 and rel_l2(o, obj) <= 1e-5
+    k0 = synth.gaussian_kernel((5, 7, 7), 1.2, 1.68)
+    e0 = rel_l2(k0, kt)
+    for outer in (1, 2):
+        o, h = _engine("generic", lambda: api.BlindRL(obs, k0, outer, 3))
+        ro, rh = oracle.blind_rl(obs, k0, outer, 3)
+        assert rel_l2(h, rh) <= 1e-4 and rel_l2(o, ro) <= 1e-4
+        print(f"[blind] recovery {outer} x 3: kernel rel L2 {rel_l2(h, kt):.3f} (start {e0:.3f})")
+        assert rel_l2(h, kt) < 0.5 * e0
+
+
+def test_blind_rl_from_initial_fused(clarity_gpu):
+    """ClarityEx_BlindRLFrom on the fused engines: a given first image
+    estimate, against the oracle from the same estimate."""
+    api = clarity_gpu
+    obs, k0 = _blind_case("beads64")
+    init = np.full(obs.shape, float(obs.mean()), np.float32)
+    o, h = _engine("fused", lambda: api.BlindRL(obs, k0, 2, 2, initial=init))
+    ro, rh = oracle.blind_rl(obs, k0, 2, 2, initial=init)
+    assert rel_l2(o, ro) <= 1e-4 and rel_l2(h, rh) <= 1e-4
+
+
+def test_divide_guard_and_quotient(clarity_gpu):
+    """The fused passes' guarded ratio: the guard exactly at the reference's
+    double compare |b| < 1e-5 (b = 0, +-1e-5f -> value; +-nextafterf(1e-5f,
+    inf) -> a / b), and the quotient within 1 ulp of IEEE float division
+    over a sweep of magnitudes (reference src/ComputePrimitives.cxx:143-148)."""
+    api = clarity_gpu
+    f32 = np.float32
+    up = np.nextafter(f32(1e-5), f32(np.inf))
+    b = np.array([0.0, -0.0, 1e-5, -1e-5, up, -up, 1e-6, 3.0], np.float32)
+    a = np.full(b.shape, 7.0, np.float32)
+    r = api.DivideArraysComponentWise(a, b, -2.5)
+    assert np.array_equal(r[:4], np.full(4, -2.5, np.float32))
+    assert r[6] == f32(-2.5)
+    exact = (a[[4, 5, 7]] / b[[4, 5, 7]]).astype(np.float32)
+    ulp = np.abs(r[[4, 5, 7]].view(np.int32).astype(np.int64)
+                 - exact.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+    # the guard's outcome is the oracle's (the reference's double compare)
+    assert np.array_equal(r == f32(-2.5), oracle.divide(a, b, -2.5) == f32(-2.5))
+    # numerators 1e-10..1e10, denominators 1e-5..1e20 of either sign:
+    # quotients 1e-30..1e15, well inside the normal range
+    rng = np.random.default_rng(17)
+    n = 1 << 20
+    sign = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    bb = (sign * 10.0 ** rng.uniform(-5, 20, n)).astype(np.float32)
+    bb = np.where(np.abs(bb) < up, np.copysign(up, bb), bb).astype(np.float32)
+    aa = (10.0 ** rng.uniform(-10, 10, n) * np.where(rng.random(n) < 0.5, -1, 1)
+          ).astype(np.float32)
+    got = api.DivideArraysComponentWise(aa, bb, 0.0)
+    want = (aa / bb).astype(np.float32)
+    fin = np.isfinite(want) & (np.abs(want) >= np.finfo(np.float32).tiny)
+    d = np.abs(got[fin].view(np.int32).astype(np.int64) - want[fin].view(np.int32).astype(np.int64))
+    print(f"[divide] {fin.sum()} quotients: max ulp {d.max()}, exact {np.mean(d == 0):.4f}")
+    assert d.max() <= 1
 
 
 def test_idivergence_vs_oracle(clarity_gpu):
