@@ -83,6 +83,7 @@ _SIGS = [
     ("ClarityEx_DivideArraysComponentWise", _R,
      [_F, _F, _F, ctypes.c_float, ctypes.c_int64]),
     ("ClarityEx_LastEngine", ctypes.c_char_p, []),
+    ("ClarityEx_ConvolveNoAlloc", _R, [_F, Dim3, _F, Dim3, _F, ctypes.c_void_p]),
     ("ClarityEx_IDivergence", _R,
      [_F, _F, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
     ("ClarityEx_RLPlanCreate", _R, [ctypes.POINTER(ctypes.c_void_p), Dim3, Dim3]),

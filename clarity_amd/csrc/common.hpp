@@ -32,6 +32,9 @@ struct Runtime {
   size_t cache_bytes = 0;
   std::map<int, float2 *> twiddles;     // w_N^j tables, j < N
   const char *last_engine = "";         // engine of the last driver call
+  // device address of the allocation-free entry point's scratch array, per
+  // device (looked up at registration; nullptr if the lookup failed)
+  std::map<int, void *> noalloc_scratch;
 };
 
 Runtime &rt();
@@ -182,6 +185,11 @@ struct PencilGeom {
 };
 
 ClarityResult_t axis_plan(int n, AxisPlan *out);
+// The factorisation of axis_plan without the twiddle table (tw = nullptr).
+ClarityResult_t axis_factor(int n, AxisPlan *out);
+// tw[j] = w_n^j for j < n, computed on the device in float64 and rounded to
+// float as axis_plan's host table is.
+ClarityResult_t k_twiddles(float2 *tw, int n, hipStream_t s);
 
 // Complex pencils, in may equal out.  sign -1 forward, +1 inverse.
 ClarityResult_t fft_c2c_pass(const float2 *in, float2 *out, const AxisPlan &ap,
@@ -199,6 +207,17 @@ ClarityResult_t fft3d_r2c(int nx, int ny, int nz, const float *in, float2 *out,
 // spec is used as scratch (overwritten), like FFTW's c2r.
 ClarityResult_t fft3d_c2r(int nx, int ny, int nz, float2 *spec, float *out,
                           hipStream_t s);
+// The same with the axes' plans (twiddle tables included) given.
+ClarityResult_t fft3d_r2c_planned(const AxisPlan &px, const AxisPlan &py,
+                                  const AxisPlan &pz, const float *in, float2 *out,
+                                  hipStream_t s);
+ClarityResult_t fft3d_c2r_planned(const AxisPlan &px, const AxisPlan &py,
+                                  const AxisPlan &pz, float2 *spec, float *out,
+                                  hipStream_t s);
+
+// Size of the per-device scratch array of the allocation-free entry point
+// (ClarityEx_ConvolveNoAlloc), a static device array in runtime.hip.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
 
 // ---------------------------------------------------------------------------
 // Pointwise kernels (pointwise.hip), all on stream s, n elements.

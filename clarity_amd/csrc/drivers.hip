@@ -491,6 +491,65 @@ ClarityResult_t divide_api(float *result, const float *a, const float *b,
   return CLARITY_SUCCESS;
 }
 
+// Clarity_Convolve on device buffers without allocating: the generic passes
+// over the reference's work domain (image + kernel per axis, kernel
+// centred by the -k/2 cyclic shift, src/Convolve.cxx:41-115,146-184), every
+// intermediate (the padded volume, both spectra, the three twiddle tables)
+// carved out of the per-device scratch array, all on one stream with no
+// host synchronisation and no memset (every region is written in full
+// before it is read).  Calls that overlap on one GPU share the array and
+// race.
+ClarityResult_t convolve_noalloc(const float *in, Clarity_Dim3 id, const float *kernel,
+                                 Clarity_Dim3 kd, float *out, void *stream) {
+  if (!in || !kernel || !out) return CLARITY_INVALID_ARGUMENT;
+  if (!dims_ok(id) || !dims_ok(kd)) return CLARITY_INVALID_ARGUMENT;
+  if (!ready()) return CLARITY_INVALID_OPERATION;
+  Runtime &r = rt();
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : r.stream;
+  auto it = r.noalloc_scratch.find(r.device);
+  if (it == r.noalloc_scratch.end() || !it->second) return CLARITY_INVALID_OPERATION;
+  unsigned char *base = static_cast<unsigned char *>(it->second);
+  const Dims I = dims_of(id), K = dims_of(kd);
+  const long long wx = (long long)I.x + K.x, wy = (long long)I.y + K.y,
+                  wz = (long long)I.z + K.z;
+  if (wx > kMaxGenericN || wy > kMaxGenericN || wz > kMaxGenericN)
+    return CLARITY_INVALID_ARGUMENT;
+  const Dims W{(int)wx, (int)wy, (int)wz};
+  AxisPlan px, py, pz;
+  CL_TRY(axis_factor(W.x, &px));
+  CL_TRY(axis_factor(W.y, &py));
+  CL_TRY(axis_factor(W.z, &pz));
+  // scratch layout, 256-byte aligned regions: padded real volume P, spectra
+  // A and B, twiddle tables for x, y, z
+  const long long V = W.voxels();
+  const long long C = (long long)W.z * W.y * (W.x / 2 + 1);
+  auto up = [](unsigned long long b) { return (b + 255ull) & ~255ull; };
+  const unsigned long long oP = 0, oA = oP + up(4ull * V), oB = oA + up(8ull * C),
+                           oX = oB + up(8ull * C), oY = oX + up(8ull * W.x),
+                           oZ = oY + up(8ull * W.y), end = oZ + up(8ull * W.z);
+  if (end > kNoAllocScratchBytes) return CLARITY_INVALID_ARGUMENT;
+  float *P = reinterpret_cast<float *>(base + oP);
+  float2 *A = reinterpret_cast<float2 *>(base + oA);
+  float2 *B = reinterpret_cast<float2 *>(base + oB);
+  px.tw = reinterpret_cast<float2 *>(base + oX);
+  py.tw = reinterpret_cast<float2 *>(base + oY);
+  pz.tw = reinterpret_cast<float2 *>(base + oZ);
+  CL_TRY(k_twiddles(reinterpret_cast<float2 *>(base + oX), W.x, s));
+  CL_TRY(k_twiddles(reinterpret_cast<float2 *>(base + oY), W.y, s));
+  CL_TRY(k_twiddles(reinterpret_cast<float2 *>(base + oZ), W.z, s));
+  const int zero[3] = {0, 0, 0};
+  const int kshift[3] = {-K.x / 2, -K.y / 2, -K.z / 2};
+  CL_TRY(k_pad_shift(P, W, in, I, zero, 0.f, s));
+  CL_TRY(fft3d_r2c_planned(px, py, pz, P, A, s));
+  CL_TRY(k_pad_shift(P, W, kernel, K, kshift, 0.f, s));
+  CL_TRY(fft3d_r2c_planned(px, py, pz, P, B, s));
+  CL_TRY(k_modulate(A, B, A, C, inv_volume(W.x, W.y, W.z), s));
+  CL_TRY(fft3d_c2r_planned(px, py, pz, A, P, s));
+  CL_TRY(k_clip(out, I, P, W, s));
+  r.last_engine = "generic";
+  return CLARITY_SUCCESS;
+}
+
 // Zero-filling stubs kept for ABI completeness (reference
 // src/SmoothedJansenVanCittertDeconvolve.cxx:28-39,
 // src/IDivergenceDeconvolve.cxx:59-72).
@@ -604,6 +663,14 @@ ClarityResult_t ClarityEx_IDivergence(const float *g, const float *ghat,
                                       int64_t n, double *result) {
   API_LOCK;
   return idiv_api(g, ghat, n, result);
+}
+
+ClarityResult_t ClarityEx_ConvolveNoAlloc(const float *inImage, Clarity_Dim3 imageDim,
+                                          const float *kernelImage,
+                                          Clarity_Dim3 kernelDim, float *outImage,
+                                          void *stream) {
+  API_LOCK;
+  return convolve_noalloc(inImage, imageDim, kernelImage, kernelDim, outImage, stream);
 }
 
 ClarityResult_t ClarityEx_DivideArraysComponentWise(float *result, const float *a,

@@ -283,8 +283,8 @@ ClarityResult_t make_args(const AxisPlan &ap, const PencilGeom &g, int sign,
 }  // namespace
 
 // Factorises n into radices 8, 4, 2 (powers of two first), then 3, 5, 7 and
-// other odd primes up to 64; builds (or reuses) the w_n table on the device.
-ClarityResult_t axis_plan(int n, AxisPlan *out) {
+// other odd primes up to 64.
+ClarityResult_t axis_factor(int n, AxisPlan *out) {
   if (n <= 0 || n > kMaxGenericN) return CLARITY_FFT_FAILED;
   AxisPlan ap;
   ap.N = n;
@@ -301,6 +301,32 @@ ClarityResult_t axis_plan(int n, AxisPlan *out) {
       m /= p;
     }
   }
+  *out = ap;
+  return CLARITY_SUCCESS;
+}
+
+namespace {
+
+__global__ void twiddle_kernel(float2 *tw, int n) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const double a = -2.0 * M_PI * (double)j / (double)n;
+    tw[j] = make_float2((float)cos(a), (float)sin(a));
+  }
+}
+
+}  // namespace
+
+ClarityResult_t k_twiddles(float2 *tw, int n, hipStream_t s) {
+  if (n <= 0) return CLARITY_INVALID_ARGUMENT;
+  const int blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(twiddle_kernel, dim3(blocks), dim3(256), 0, s, tw, n);
+  return launch_check();
+}
+
+// axis_factor, then builds (or reuses) the w_n table on the device.
+ClarityResult_t axis_plan(int n, AxisPlan *out) {
+  AxisPlan ap;
+  CL_TRY(axis_factor(n, &ap));
   Runtime &r = rt();
   auto it = r.twiddles.find(n);
   if (it == r.twiddles.end()) {
@@ -365,6 +391,13 @@ ClarityResult_t fft3d_r2c(int nx, int ny, int nz, const float *in, float2 *out,
   CL_TRY(axis_plan(nx, &px));
   CL_TRY(axis_plan(ny, &py));
   CL_TRY(axis_plan(nz, &pz));
+  return fft3d_r2c_planned(px, py, pz, in, out, s);
+}
+
+ClarityResult_t fft3d_r2c_planned(const AxisPlan &px, const AxisPlan &py,
+                                  const AxisPlan &pz, const float *in, float2 *out,
+                                  hipStream_t s) {
+  const int nx = px.N, ny = py.N, nz = pz.N;
   const long long hx = nx / 2 + 1;
   PencilGeom gx;  // rows (z, y): a = z, b = y
   gx.nA = nz; gx.nB = ny;
@@ -390,6 +423,13 @@ ClarityResult_t fft3d_c2r(int nx, int ny, int nz, float2 *spec, float *out,
   CL_TRY(axis_plan(nx, &px));
   CL_TRY(axis_plan(ny, &py));
   CL_TRY(axis_plan(nz, &pz));
+  return fft3d_c2r_planned(px, py, pz, spec, out, s);
+}
+
+ClarityResult_t fft3d_c2r_planned(const AxisPlan &px, const AxisPlan &py,
+                                  const AxisPlan &pz, float2 *spec, float *out,
+                                  hipStream_t s) {
+  const int nx = px.N, ny = py.N, nz = pz.N;
   const long long hx = nx / 2 + 1;
   PencilGeom gz;
   gz.nA = 1; gz.nB = (long long)ny * hx;

@@ -8,6 +8,14 @@
 
 namespace clarity {
 
+// Scratch of the allocation-free entry point (ClarityEx_ConvolveNoAlloc):
+// every GPU the library runs on holds its own copy of this array until the
+// process ends.  Host code never passes the array's name to a kernel (on
+// the host it only stands for the symbol); it uses the device address that
+// Clarity_Register looks up with hipGetSymbolAddress, once per device, and
+// keeps in Runtime::noalloc_scratch.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 Runtime &rt() {
   static Runtime r;
   return r;
@@ -126,6 +134,17 @@ ClarityResult_t Clarity_Register() {
   if (const char *s = std::getenv("CLARITY_DEVICE")) dev = std::atoi(s);
   if (dev < 0 || dev >= count) return CLARITY_INVALID_ARGUMENT;
   CL_HIP(hipSetDevice(dev));
+  // the scratch array's address on this device (the device is current);
+  // a failed lookup leaves nullptr, and only the allocation-free entry
+  // point, which then returns CLARITY_INVALID_OPERATION, depends on it
+  if (r.noalloc_scratch.find(dev) == r.noalloc_scratch.end()) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      ptr = nullptr;
+    }
+    r.noalloc_scratch[dev] = ptr;
+  }
   hipDeviceProp_t prop;
   CL_HIP(hipGetDeviceProperties(&prop, dev));
   CL_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking));

@@ -198,6 +198,30 @@ C_FUNC_DEF ClarityResult_t ClarityEx_DivideArraysComponentWise(float *result,
                                                                float value,
                                                                int64_t n);
 
+/* Clarity_Convolve on DEVICE buffers with no allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * inImage (imageDim floats), kernelImage (kernelDim floats) and outImage
+ * (imageDim floats) are device pointers; the result is Clarity_Convolve's
+ * (reference work domain image + kernel per axis, generic FFT passes).
+ * Enqueued on `stream` (a hipStream_t; NULL = the library stream) with no
+ * host synchronisation: synchronise that stream before reading outImage.
+ * Every intermediate lives in one static per-device scratch array of 32 MiB
+ * (held by each GPU the library runs on until the process ends): the work
+ * domain (imageDim + kernelDim per axis, each axis <= 4096 with prime
+ * factors <= 64) must satisfy 4 V + 16 C + 8 (nx + ny + nz) <= 32 MiB
+ * (V = nx ny nz, C = nz ny (nx/2 + 1), each term rounded up to 256 bytes),
+ * e.g. a 128x128x64 image with a 32x32x32 kernel (work 160x160x96); larger
+ * shapes return CLARITY_INVALID_ARGUMENT, other axis lengths
+ * CLARITY_FFT_FAILED, before anything is enqueued.  Calls share that array:
+ * calls that overlap on one GPU (other streams, other threads' streams)
+ * race, and the array keeps the last call's contents. */
+C_FUNC_DEF ClarityResult_t ClarityEx_ConvolveNoAlloc(const float *inImage,
+                                                     Clarity_Dim3 imageDim,
+                                                     const float *kernelImage,
+                                                     Clarity_Dim3 kernelDim,
+                                                     float *outImage, void *stream);
+
 /* Engine of the last Convolve, Wiener, Jansen-van Cittert, (blind) ML or
  * blind RL call: "fast" (the fused passes) or "generic" ("" before any). */
 C_FUNC_DEF const char *ClarityEx_LastEngine();
