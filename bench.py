@@ -18,8 +18,11 @@ Also reported:
                 recording them costs ~4 % of the iteration time);
                 algorithmic bytes per launch from the plan's own model.
   cpu_baseline  the CPU oracle (the reference's CPU algorithm restated in C,
-                FFTW3-API FFTs via MKL when present, plan per FFT, float32
-                serial energy sum), timed on this host on a bounded sample.
+                FFTW3-API FFTs via MKL when present, plan per FFT), 10
+                iterations of the same volume on this host's OpenMP threads,
+                both reduction modes, deconvolution and whole-call scopes,
+                median of 3 after a warm-up; value = serial float32 sum
+                (the reference's), deconvolution scope.
   learned_warm_start
                 the metric's second half: the committed network + 20 RL
                 iterations against plain RL on the same volume, measured against
@@ -109,23 +112,55 @@ def make_inputs(api, shape, seed=3):
     return obs, psf
 
 
+def host_cpu():
+    """lscpu's model, sockets, cores per socket, threads per core and the
+    logical CPU count of this host."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets",
+            "Core(s) per socket": "cores_per_socket", "Thread(s) per core": "threads_per_core",
+            "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys and keys[k.strip()] not in info:
+            info[keys[k.strip()]] = v.strip()
+    return info
+
+
 def cpu_baseline(obs, psf, iterations, threads):
-    """The oracle's RL on this host: `iterations` iterations of the full
-    volume, per-iteration loop time.  Reference cost structure (FFTW3 API
-    with a plan per call, buffer per convolution, float32 serial sum)."""
+    """The oracle's RL on this host (the reference's CPU algorithm: FFTW3 API
+    with a plan per call, a buffer per convolution), `iterations`
+    iterations of the full volume, in both reduction modes (the reference's
+    sequential float32 energy sum, and float64).  Per mode one warm-up call
+    of one iteration, then the median of 3 calls, in the reference's two
+    timing scopes: deconvolution only (TIME_DECONVOLVE, after padding and
+    before clipping) and the whole call (TIME_TOTAL)."""
     import oracle
     backend = "own-fp64"
     if oracle.set_fft_backend(oracle.FFT_FFTW_API):
         backend = oracle.fft_backend_name()
-    oracle.set_reduce_mode(oracle.REDUCE_SERIAL_F32)
     oracle.set_threads(threads)
-    t0 = time.perf_counter()
-    oracle.ml(obs, psf, iterations)
-    total = time.perf_counter() - t0
-    loop = oracle.last_loop_seconds()
+    res = {}
+    for mode, name in ((oracle.REDUCE_SERIAL_F32, "serial_f32"), (oracle.REDUCE_FP64, "fp64")):
+        oracle.set_reduce_mode(mode)
+        oracle.ml(obs, psf, 1)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.ml(obs, psf, iterations)
+            total = time.perf_counter() - t0
+            runs.append((oracle.last_deconvolve_seconds(), total, oracle.last_loop_seconds()))
+            log(f"[bench] cpu {name}: deconvolve {runs[-1][0]:.2f} s, total {total:.2f} s")
+        med = [float(np.median([r[i] for r in runs])) for i in range(3)]
+        res[name] = {"deconvolve_s": round(med[0], 3), "total_s": round(med[1], 3),
+                     "iterations_s": round(med[2], 3)}
     oracle.set_reduce_mode(oracle.REDUCE_FP64)
     oracle.set_fft_backend(oracle.FFT_OWN)
-    return loop, total, backend
+    return res, backend
 
 
 def dense_bytes(shape, passes_real, passes_complex):
@@ -254,7 +289,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iterations", type=int, default=100)
     ap.add_argument("--shape", type=str, default="128,512,512", help="image z,y,x")
-    ap.add_argument("--cpu-iterations", type=int, default=1)
+    ap.add_argument("--cpu-iterations", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned warm start's time-to-accuracy measurement")
@@ -370,16 +405,27 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline: {args.cpu_iterations} iteration(s) on {threads} threads")
-        loop, total, backend = cpu_baseline(obs, psf, args.cpu_iterations, threads)
-        cpu = {"value": round(vox * args.cpu_iterations / loop / 1e6, 3),
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        hostinfo = host_cpu()
+        log(f"[bench] cpu baseline: {args.cpu_iterations} iterations on {threads} threads "
+            f"({hostinfo})")
+        res, backend = cpu_baseline(obs, psf, args.cpu_iterations, threads)
+        rate = {k: {"deconvolve": round(vox * args.cpu_iterations / v["deconvolve_s"] / 1e6, 3),
+                    "total": round(vox * args.cpu_iterations / v["total_s"] / 1e6, 3),
+                    "iterations_only": round(vox * args.cpu_iterations / v["iterations_s"] / 1e6,
+                                             3)}
+                for k, v in res.items()}
+        cpu = {"value": rate["serial_f32"]["deconvolve"],
                "unit": "Mvoxel-iters/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_iterations} RL iteration(s) of the same "
+               "sample": f"{args.cpu_iterations} RL iterations of the same "
                          f"{shape[2]}x{shape[1]}x{shape[0]} volume (work "
-                         f"{nx}x{ny}x{nz}), loop time {loop:.2f} s of a "
-                         f"{total:.2f} s call; FFT {backend}, serial float32 sum"}
-        log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s")
+                         f"{nx}x{ny}x{nz}); value = the reference's arithmetic (serial "
+                         f"float32 energy sum), deconvolution scope (TIME_DECONVOLVE: "
+                         f"kernel transform, energy, iterations), median of 3 after one "
+                         f"warm-up call; FFT {backend} on {threads} threads",
+               "Mvoxel_iters_per_s": rate, "seconds": res, "host": hostinfo,
+               "fft": backend}
+        log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s; {rate}")
 
     # time-to-accuracy of the learned warm start against plain RL, on the
     # same volume (network + 20 RL iterations vs plain RL, measured against
@@ -406,6 +452,18 @@ def main():
                 else round(ev["time_to_accuracy_speedup"], 2),
                 "wall_ratio_vs_rl200": round(ev["wall_ratio_vs_yardstick"], 2),
                 "accepted_at_1e-3": ev["accepted"],
+                "at_1e-3": {
+                    "plain_iterations": ev["plain_iterations_to_tolerance"],
+                    "plain_ms": None if ev["plain_seconds_to_tolerance"] is None
+                    else round(ev["plain_seconds_to_tolerance"] * 1e3, 2),
+                    "learned_refine_iterations": ev["learned_refine_iterations_to_tolerance"],
+                    "learned_ms": None if ev["learned_seconds_to_tolerance"] is None
+                    else round(ev["learned_seconds_to_tolerance"] * 1e3, 2),
+                    "learned_best_rel_l2": round(
+                        ev["learned_best_rel_l2_within_yardstick_iterations"], 5),
+                    "speedup": None if ev["speedup_at_tolerance"] is None
+                    else round(ev["speedup_at_tolerance"], 2),
+                },
             }
             log(f"[bench] learned {learned}")
 

@@ -525,7 +525,6 @@ struct VolTab {
   Geom g;
   Bufs b;
   int blk0;    // the volume's first block in the launch
-  int conj;    // z_mod: product with the kernel column's conjugate
   int nparts;  // finish: partial sums of the volume's last x pass
 };
 
@@ -889,7 +888,9 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
   // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
   // |s1| < 1e-5 (the reference compares in double; kDivGuard is the same
   // test in float) gives 0, as the reference's ML step
-  // (src/ComputePrimitives.cxx:143-148, src/MaximumLikelihoodDeconvolve.cxx:62)
+  // (src/ComputePrimitives.cxx:143-148, src/MaximumLikelihoodDeconvolve.cxx:62).
+  // This is guarded_ratio(o, s, 0) written out: the compiler allocates fewer
+  // registers for the x pass with the expression inline.
   const float c = cptr[0];
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -897,11 +898,11 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
     float r0 = 0.f, r1 = 0.f;
     if (valid && in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].x;
-      r0 = guarded_ratio(o[e].x, s, 0.f);
+      r0 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].x, s);
     }
     if (valid && second_in_image<HE>(e, n0, g.ix)) {
       const float s = c * v[e].y;
-      r1 = guarded_ratio(o[e].y, s, 0.f);
+      r1 = (fabsf(s) < kDivGuard) ? 0.f : quotient(o[e].y, s);
     }
     v[e] = make_float2(r0, r1);
   }
@@ -1242,9 +1243,9 @@ __global__ void __launch_bounds__(PPW *(N / E))
 // and written are exactly elements e < HZ, and the kernel's are e < HZ/2
 // (planes z < kz/2) and e >= E - HZ/2 (planes z >= nz - kz/2): no bounds
 // tests.
-template <int N, int E, int TP, int HZ, bool WIENER = false>
+template <int N, int E, int TP, int HZ, bool WIENER = false, bool CONJ = false>
 __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, int conjk, float nn2 = 0.f, float eps = 0.f) {
+        const float2 *__restrict__ gtw, Geom g, float nn2 = 0.f, float eps = 0.f) {
   constexpr bool H = HZ > 0;
   static_assert(HZ % 2 == 0 && HZ <= E, "kernel planes in whole elements");
   constexpr int T = N / E;
@@ -1318,11 +1319,12 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
                 KO = HALF ? kKeepLower : 0;
   fft_pruned<N, E, 1, -1, ZV, 0>(v, t, L, tw, typename Plan<N, E>::Rad{});
   fft_pruned<N, E, 1, -1, ZH, 0>(h, t, L, tw, typename Plan<N, E>::Rad{});
-  if (conjk) {
-    // a correlation: the product with conj of the kernel column's 3D
-    // spectrum (the negation is exact)
+  // a correlation (CONJ): the product with conj of the kernel column's 3D
+  // spectrum, taken as conj(conj(S) H) so that only the spectrum's
+  // registers change (exact sign flips)
+  if constexpr (CONJ) {
 #pragma unroll
-    for (int e = 0; e < E; e++) h[e].y = -h[e].y;
+    for (int e = 0; e < E; e++) v[e].y = -v[e].y;
   }
   if constexpr (WIENER) {
     // reference Wiener filter (src/WienerDeconvolve.cxx:54-64): Hc =
@@ -1340,6 +1342,10 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
 #pragma unroll
     for (int e = 0; e < E; e++) v[e] = cmul(v[e], h[e]);
   }
+  if constexpr (CONJ) {
+#pragma unroll
+    for (int e = 0; e < E; e++) v[e].y = -v[e].y;
+  }
   fft_pruned<N, E, 1, +1, 0, KO>(v, t, L, tw, typename Plan<N, E>::Rad{});
   {
     float2 *p = s0;
@@ -1354,24 +1360,25 @@ __device__ __forceinline__ void z_body(int blk, float2 *__restrict__ S, const fl
   }
 }
 
-template <int N, int E, int TP, int HZ>
+template <int N, int E, int TP, int HZ, bool CONJ = false>
 __global__ void __launch_bounds__(TP *(N / E))
     k_z(float2 *__restrict__ S, const float2 *__restrict__ K,
-        const float2 *__restrict__ gtw, Geom g, int conjk) {
-  z_body<N, E, TP, HZ>(blockIdx.x, S, K, gtw, g, conjk);
+        const float2 *__restrict__ gtw, Geom g) {
+  z_body<N, E, TP, HZ, false, CONJ>(blockIdx.x, S, K, gtw, g);
 }
+// batched: RL volumes only (make_rl_group takes no kernel-step engine)
 template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
   const VolTab &v = vt[vmap[blockIdx.x]];
-  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g, v.conj);
+  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g);
 }
 // z pass of the Wiener filter (single volume)
 template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
     k_zw(float2 *__restrict__ S, const float2 *__restrict__ K, const float2 *__restrict__ gtw,
          Geom g, float nn2, float eps) {
-  z_body<N, E, TP, HZ, true>(blockIdx.x, S, K, gtw, g, 0, nn2, eps);
+  z_body<N, E, TP, HZ, true>(blockIdx.x, S, K, gtw, g, nn2, eps);
 }
 
 // The same-size z pass (iz = kz = N/2) over several column tiles per
@@ -1690,9 +1697,14 @@ void launch_z_hz(const ZOp &op, const Geom &g, const Bufs &b, unsigned grid, hip
                        b.twZ, g, op.nn2, op.eps);
     return;
   }
+  if (op.conj) {
+    hipLaunchKernelGGL((k_z<N, E, TP, HZ, true>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S,
+                       b.K, b.twZ, g);
+    return;
+  }
   if (recorded(&kb_z<N, E, TP, HZ>, grid, TP * (N / E))) return;
   hipLaunchKernelGGL((k_z<N, E, TP, HZ>), dim3(grid), dim3(TP * (N / E)), 0, s, b.S, b.K,
-                     b.twZ, g, (int)op.conj);
+                     b.twZ, g);
 }
 
 template <int N, int E, int TP>
@@ -1860,7 +1872,7 @@ class FastRL : public RLEngine {
 
   // The batched form of the launch `kind` makes (z_mod: of the iteration's
   // second half when `second`), and this engine's entry of its table.
-  LaunchRec batched(Kind kind, bool second) const {
+  LaunchRec batched(Kind kind) const {
     LaunchRec r{nullptr, 0, 0};
     g_rec = &r;
     switch (kind) {
@@ -1870,20 +1882,13 @@ class FastRL : public RLEngine {
       case KX_LAST_FWD: dispatch_x(tu_, kind, g_, b_, nullptr); break;
       case KY_FWD: dispatch_y(tu_, true, g_, b_, nullptr); break;
       case KY_INV: dispatch_y(tu_, false, g_, b_, nullptr); break;
-      case KZ_MOD: {
-        ZOp op;
-        op.conj = corr_ && second;
-        dispatch_z(tu_, g_, b_, nullptr, op);
-        break;
-      }
+      case KZ_MOD: dispatch_z(tu_, g_, b_, nullptr, ZOp{}); break;
       default: break;
     }
     g_rec = nullptr;
     return r;
   }
-  VolTab table_entry(bool second) const {
-    return VolTab{g_, b_, 0, (corr_ && second) ? 1 : 0, nparts_};
-  }
+  VolTab table_entry() const { return VolTab{g_, b_, 0, nparts_}; }
 
   // The kernel half-step of alternating blind RL: the correction a
   // correlation (the blurring input's conjugate spectrum), the estimate
@@ -1908,6 +1913,7 @@ class FastRL : public RLEngine {
   bool is_output() const { return out_; }
   bool is_jvc() const { return jvc_; }
   bool is_unscaled() const { return norescale_; }
+  bool is_kernel_step() const { return corr_; }
   ClarityResult_t set_jvc_constants(float A, float invASq, hipStream_t s) override {
     CL_TRY(k_fill(scal_.f(), 1, A, s));
     return k_fill(scal_.f() + 1, 1, invASq, s);
@@ -2375,7 +2381,7 @@ class FastGroup : public RLGroup {
         // volumes by (batched kernel, block size), in volume order
         std::vector<std::pair<LaunchRec, std::vector<int>>> groups;
         for (int v = 0; v < (int)e_.size(); v++) {
-          const LaunchRec r = e_[v]->batched(k, half == 1);
+          const LaunchRec r = e_[v]->batched(k);
           if (!r.fn || r.grid == 0) return CLARITY_INVALID_OPERATION;
           auto it = groups.begin();
           for (; it != groups.end(); ++it)
@@ -2393,9 +2399,9 @@ class FastGroup : public RLGroup {
           std::vector<VolTab> tab;
           std::vector<unsigned> map;
           for (int v : gr.second) {
-            VolTab t = e_[v]->table_entry(half == 1);
+            VolTab t = e_[v]->table_entry();
             t.blk0 = (int)map.size();
-            const unsigned grid = e_[v]->batched(k, half == 1).grid;
+            const unsigned grid = e_[v]->batched(k).grid;
             map.insert(map.end(), grid, (unsigned)tab.size());
             tab.push_back(t);
           }
@@ -2410,7 +2416,7 @@ class FastGroup : public RLGroup {
         }
       }
     std::vector<VolTab> fin;
-    for (FastRL *e : e_) fin.push_back(e->table_entry(false));
+    for (FastRL *e : e_) fin.push_back(e->table_entry());
     CL_TRY(fin_.alloc(sizeof(VolTab) * fin.size()));
     CL_HIP(hipMemcpy(fin_.p, fin.data(), sizeof(VolTab) * fin.size(), hipMemcpyHostToDevice));
     return CLARITY_SUCCESS;
@@ -2642,7 +2648,8 @@ std::unique_ptr<RLGroup> make_rl_group(const std::vector<RLEngine *> &engines) {
   std::vector<FastRL *> f;
   for (RLEngine *e : engines) {
     auto *x = dynamic_cast<FastRL *>(e);
-    if (!x || x->is_jvc() || x->is_unscaled() || x->is_output()) return nullptr;
+    if (!x || x->is_jvc() || x->is_unscaled() || x->is_output() || x->is_kernel_step())
+      return nullptr;
     f.push_back(x);
   }
   if (f.empty() || f.size() > 65535) return nullptr;

@@ -295,6 +295,40 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
         done = k
         plan.estimate_device(cur.data_ptr())
         curve.append((k, err(cur)))
+    # at the stated tolerance (ACCEPT_REL_L2 of the yardstick): the first
+    # plain-RL iteration count that reaches it (one by one near the end, where
+    # the curve crosses it), and the first refinement count after the
+    # network that reaches it (every 10 iterations, up to the yardstick's
+    # count)
+    tol = ACCEPT_REL_L2
+    k_plain_tol = None
+    start = max(0, YARDSTICK_ITERATIONS - 60)
+    for k, e in curve:
+        if e <= tol:
+            start = max(0, k - curve_step)
+            break
+    plan.reset()
+    plan.iterate(start)
+    for k in range(start + 1, YARDSTICK_ITERATIONS + 1):
+        plan.iterate(1)
+        plan.estimate_device(cur.data_ptr())
+        if err(cur) <= tol:
+            k_plain_tol = k
+            break
+    net.predict_device(obs_d.data_ptr(), shape, sigma_xy, sigma_z, warm.data_ptr())
+    plan.reset_device(warm.data_ptr())
+    k_learned_tol, best_learned, learned_curve = None, float("inf"), []
+    done = 0
+    for k in range(curve_step, YARDSTICK_ITERATIONS + 1, curve_step):
+        plan.iterate(k - done)
+        done = k
+        plan.estimate_device(cur.data_ptr())
+        e = err(cur)
+        learned_curve.append((k, e))
+        best_learned = min(best_learned, e)
+        if e <= tol:
+            k_learned_tol = k
+            break
     learned()
     plan.estimate_device(cur.data_ptr())
     e_learned = err(cur)
@@ -316,6 +350,8 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
         prev = (k, e)
     per_it = t_plain / YARDSTICK_ITERATIONS
     t_eq = None if k_eq is None else k_eq * per_it
+    t_plain_tol = None if k_plain_tol is None else k_plain_tol * per_it
+    t_learned_tol = None if k_learned_tol is None else t_net + k_learned_tol * per_it
     ev = {
         "volume": list(shape),
         "sigma_xy": sigma_xy, "sigma_z": sigma_z,
@@ -330,6 +366,15 @@ def evaluate(net, case=None, sigma_xy: float = 2.0, sigma_z: float = 4.5,
         "wall_ratio_vs_yardstick": t_plain / t_learned,
         "plain_iterations_to_same_error": k_eq,
         "time_to_accuracy_speedup": None if t_eq is None else t_eq / t_learned,
+        "tolerance": tol,
+        "plain_iterations_to_tolerance": k_plain_tol,
+        "plain_seconds_to_tolerance": t_plain_tol,
+        "learned_refine_iterations_to_tolerance": k_learned_tol,
+        "learned_seconds_to_tolerance": t_learned_tol,
+        "learned_best_rel_l2_within_yardstick_iterations": best_learned,
+        "learned_curve": learned_curve,
+        "speedup_at_tolerance": (None if t_plain_tol is None or t_learned_tol is None
+                                 else t_plain_tol / t_learned_tol),
         "accepted": bool(e_learned <= ACCEPT_REL_L2 and
                          (idiv_l - idiv_p) / idiv_p <= ACCEPT_IDIV),
     }

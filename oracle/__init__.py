@@ -73,6 +73,7 @@ def lib():
         L.oracle_idivergence.restype = ctypes.c_double
         L.oracle_fft_backend_name.restype = ctypes.c_char_p
         L.oracle_last_loop_seconds.restype = ctypes.c_double
+        L.oracle_last_deconvolve_seconds.restype = ctypes.c_double
         L.oracle_set_fft_backend.argtypes = [ci]
         L.oracle_set_reduce_mode.argtypes = [ci]
         L.oracle_set_threads.argtypes = [ci]
@@ -302,6 +303,12 @@ def blind_rl(img, psf, outer, inner, initial=None):
 
 def last_loop_seconds() -> float:
     return float(lib().oracle_last_loop_seconds())
+
+
+def last_deconvolve_seconds() -> float:
+    """The last RL call's deconvolution scope (the reference's
+    TIME_DECONVOLVE: after padding, before clipping)."""
+    return float(lib().oracle_last_deconvolve_seconds())
 
 
 def idivergence(g, ghat) -> float:

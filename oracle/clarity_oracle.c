@@ -33,6 +33,11 @@ static int g_reduce_mode = ORACLE_REDUCE_FP64;
 static int g_fft_backend = ORACLE_FFT_OWN;
 static int g_threads = 0; /* 0 = OpenMP default */
 static double g_loop_seconds = 0.0; /* wall time of the last RL loop */
+/* wall time of the last RL call's deconvolution scope: from after padding to
+ * before the clip (src/MaximumLikelihoodDeconvolve.cxx:294-316,
+ * TIME_DECONVOLVE: the CPU driver's buffers, the kernel transform, the
+ * energy and the iterations) */
+static double g_deconv_seconds = 0.0;
 
 static double now_seconds(void) {
   struct timespec ts;
@@ -41,6 +46,7 @@ static double now_seconds(void) {
 }
 
 double oracle_last_loop_seconds(void) { return g_loop_seconds; }
+double oracle_last_deconvolve_seconds(void) { return g_deconv_seconds; }
 
 void oracle_set_reduce_mode(int mode) { g_reduce_mode = mode; }
 int oracle_get_reduce_mode(void) { return g_reduce_mode; }
@@ -776,6 +782,7 @@ int oracle_ml_ex(const float *in, int ix, int iy, int iz, const float *kernel,
   if (iterations < 0) return OR_INVALID_ARGUMENT;
   work_t w;
   int rc = work_setup(&w, in, ix, iy, iz, kernel, kx, ky, kz);
+  const double td0 = now_seconds();
   float *otf = NULL, *g = NULL, *s1 = NULL, *s2 = NULL;
   if (rc == OR_SUCCESS) {
     otf = (float *)malloc(sizeof(float) * (size_t)w.c);
@@ -802,6 +809,7 @@ int oracle_ml_ex(const float *in, int ix, int iy, int iz, const float *kernel,
                      scale_trace ? scale_trace + k : NULL);
     g_loop_seconds = now_seconds() - t0;
   }
+  g_deconv_seconds = now_seconds() - td0;
   if (rc == OR_SUCCESS) oracle_clip(out, ix, iy, iz, g, w.nx, w.ny, w.nz);
   free(otf);
   free(g);

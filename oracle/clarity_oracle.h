@@ -111,6 +111,9 @@ double oracle_idivergence(const float *g, const float *ghat, int64_t n);
 /* Wall seconds of the iteration loop of the last oracle_ml / oracle_ml_ex
  * call (the reference's TIME_DECONVOLVE scope minus the OTF and energy). */
 double oracle_last_loop_seconds(void);
+/* Wall seconds of the last RL call's deconvolution scope (after padding,
+ * before clipping). */
+double oracle_last_deconvolve_seconds(void);
 
 #ifdef __cplusplus
 }
