@@ -31,6 +31,7 @@ struct Runtime {
   std::multimap<size_t, void *> cache;  // freed device blocks by size
   size_t cache_bytes = 0;
   std::map<int, float2 *> twiddles;     // w_N^j tables, j < N
+  std::map<long long, float2 *> tables; // four-step and Bluestein tables
   const char *last_engine = "";         // engine of the last driver call
   // device address of the allocation-free entry point's scratch array, per
   // device (looked up at registration; nullptr if the lookup failed)
@@ -207,6 +208,24 @@ ClarityResult_t fft3d_r2c(int nx, int ny, int nz, const float *in, float2 *out,
 // spec is used as scratch (overwritten), like FFTW's c2r.
 ClarityResult_t fft3d_c2r(int nx, int ny, int nz, float2 *spec, float *out,
                           hipStream_t s);
+// Axis lengths the 3D transforms accept: every n <= kMaxFftLength.  Lengths
+// the LDS passes take in one tile (<= kMaxGenericN, prime factors <= 64)
+// run there; the others by four-step or Bluestein (fft_large.hip).
+constexpr long long kMaxFftLength = 1LL << 23;
+bool fft_length_supported(long long n);
+bool fft_length_direct(int n);
+// What fft_axis_large gathers: complex elements, real elements (imaginary
+// part 0), or a half spectrum of n/2 + 1 bins extended by Hermitian
+// symmetry (the imaginary parts of bin 0 and, for even n, bin n/2 ignored,
+// as a c2r does).
+enum FftGather { kGatherComplex = 0, kGatherReal = 1, kGatherHalf = 2 };
+// One axis of length n of any length over the pencils of g (input element
+// m of pencil (a, b) at a ia + b ib + m ie, output element k at a oa + b ob
+// + k oe): gather, transform with `sign`, scatter the first `count` outputs
+// (their real parts when real_out).  in may equal out.  Synchronises s.
+ClarityResult_t fft_axis_large(const void *in, void *out, const PencilGeom &g, int n,
+                               int kind, int sign, int count, bool real_out, hipStream_t s);
+
 // The same with the axes' plans (twiddle tables included) given.
 ClarityResult_t fft3d_r2c_planned(const AxisPlan &px, const AxisPlan &py,
                                   const AxisPlan &pz, const float *in, float2 *out,

@@ -21,7 +21,7 @@ ClarityResult_t work_geometry(Work &w, const float *in, Clarity_Dim3 id,
   w.kd = dims_of(kd);
   long long wx = (long long)id.x + kd.x, wy = (long long)id.y + kd.y,
             wz = (long long)id.z + kd.z;
-  if (wx > kMaxGenericN || wy > kMaxGenericN || wz > kMaxGenericN)
+  if (!fft_length_supported(wx) || !fft_length_supported(wy) || !fft_length_supported(wz))
     return CLARITY_FFT_FAILED;
   w.wd = Dims{(int)wx, (int)wy, (int)wz};
   w.IV = w.id.voxels();

@@ -1,5 +1,5 @@
 // Generic mixed-radix Stockham FFT passes for any axis length N <= 4096
-// whose prime factors are <= 64.  One workgroup transforms a tile of TP
+// whose prime factors are <= 64 (other lengths: fft_large.hip).  One workgroup transforms a tile of TP
 // pencils held interleaved in LDS (element j of pencil q at j*TP + q), so
 // that loads and stores of pencils that are adjacent in memory coalesce.
 //
@@ -282,6 +282,11 @@ ClarityResult_t make_args(const AxisPlan &ap, const PencilGeom &g, int sign,
 
 }  // namespace
 
+bool fft_length_direct(int n) {
+  AxisPlan ap;
+  return axis_factor(n, &ap) == CLARITY_SUCCESS;
+}
+
 // Factorises n into radices 8, 4, 2 (powers of two first), then 3, 5, 7 and
 // other odd primes up to 64.
 ClarityResult_t axis_factor(int n, AxisPlan *out) {
@@ -384,14 +389,67 @@ ClarityResult_t fft_c2r_rows(const float2 *in, float *out, const AxisPlan &ap,
   return launch_check();
 }
 
-// FFTW layout: real (z, y, x) -> spectrum (z, y, kx), hx = nx/2+1.
+namespace {
+
+// Pencils of the y and z passes over a half spectrum (z, y, kx).
+PencilGeom y_pencils(int ny, int nz, long long hx) {
+  PencilGeom g;  // pencils (z, kx) along y
+  g.nA = nz; g.nB = hx;
+  g.ia = g.oa = (long long)ny * hx; g.ib = g.ob = 1; g.ie = g.oe = hx;
+  g.Lin = ny; g.Lout = ny;
+  return g;
+}
+
+PencilGeom z_pencils(int ny, int nz, long long hx) {
+  PencilGeom g;  // pencils (y, kx) along z: one outer block
+  g.nA = 1; g.nB = (long long)ny * hx;
+  g.ia = g.oa = 0; g.ib = g.ob = 1; g.ie = g.oe = (long long)ny * hx;
+  g.Lin = nz; g.Lout = nz;
+  return g;
+}
+
+// A complex pass along y or z over the spectrum in place: the LDS passes
+// for their lengths, fft_axis_large for the others.
+ClarityResult_t spectrum_pass(float2 *spec, int n, const PencilGeom &g, int sign,
+                              hipStream_t s) {
+  if (fft_length_direct(n)) {
+    AxisPlan ap;
+    CL_TRY(axis_plan(n, &ap));
+    return fft_c2c_pass(spec, spec, ap, g, sign, s);
+  }
+  return fft_axis_large(spec, spec, g, n, kGatherComplex, sign, n, false, s);
+}
+
+}  // namespace
+
+// FFTW layout: real (z, y, x) -> spectrum (z, y, kx), hx = nx/2+1.  Every
+// axis length up to kMaxFftLength.
 ClarityResult_t fft3d_r2c(int nx, int ny, int nz, const float *in, float2 *out,
                           hipStream_t s) {
-  AxisPlan px, py, pz;
-  CL_TRY(axis_plan(nx, &px));
-  CL_TRY(axis_plan(ny, &py));
-  CL_TRY(axis_plan(nz, &pz));
-  return fft3d_r2c_planned(px, py, pz, in, out, s);
+  if (!fft_length_supported(nx) || !fft_length_supported(ny) || !fft_length_supported(nz))
+    return CLARITY_FFT_FAILED;
+  if (fft_length_direct(nx) && fft_length_direct(ny) && fft_length_direct(nz)) {
+    AxisPlan px, py, pz;
+    CL_TRY(axis_plan(nx, &px));
+    CL_TRY(axis_plan(ny, &py));
+    CL_TRY(axis_plan(nz, &pz));
+    return fft3d_r2c_planned(px, py, pz, in, out, s);
+  }
+  const long long hx = nx / 2 + 1;
+  PencilGeom gx;  // rows (z, y): a = z, b = y
+  gx.nA = nz; gx.nB = ny;
+  gx.ia = (long long)ny * nx; gx.ib = nx; gx.ie = 1;
+  gx.oa = (long long)ny * hx; gx.ob = hx; gx.oe = 1;
+  gx.Lin = nx; gx.Lout = (int)hx;
+  if (fft_length_direct(nx)) {
+    AxisPlan px;
+    CL_TRY(axis_plan(nx, &px));
+    CL_TRY(fft_r2c_rows(in, out, px, gx, s));
+  } else {
+    CL_TRY(fft_axis_large(in, out, gx, nx, kGatherReal, -1, (int)hx, false, s));
+  }
+  CL_TRY(spectrum_pass(out, ny, y_pencils(ny, nz, hx), -1, s));
+  return spectrum_pass(out, nz, z_pencils(ny, nz, hx), -1, s);
 }
 
 ClarityResult_t fft3d_r2c_planned(const AxisPlan &px, const AxisPlan &py,
@@ -419,11 +477,29 @@ ClarityResult_t fft3d_r2c_planned(const AxisPlan &px, const AxisPlan &py,
 
 ClarityResult_t fft3d_c2r(int nx, int ny, int nz, float2 *spec, float *out,
                           hipStream_t s) {
-  AxisPlan px, py, pz;
-  CL_TRY(axis_plan(nx, &px));
-  CL_TRY(axis_plan(ny, &py));
-  CL_TRY(axis_plan(nz, &pz));
-  return fft3d_c2r_planned(px, py, pz, spec, out, s);
+  if (!fft_length_supported(nx) || !fft_length_supported(ny) || !fft_length_supported(nz))
+    return CLARITY_FFT_FAILED;
+  if (fft_length_direct(nx) && fft_length_direct(ny) && fft_length_direct(nz)) {
+    AxisPlan px, py, pz;
+    CL_TRY(axis_plan(nx, &px));
+    CL_TRY(axis_plan(ny, &py));
+    CL_TRY(axis_plan(nz, &pz));
+    return fft3d_c2r_planned(px, py, pz, spec, out, s);
+  }
+  const long long hx = nx / 2 + 1;
+  CL_TRY(spectrum_pass(spec, nz, z_pencils(ny, nz, hx), +1, s));
+  CL_TRY(spectrum_pass(spec, ny, y_pencils(ny, nz, hx), +1, s));
+  PencilGeom gx;
+  gx.nA = nz; gx.nB = ny;
+  gx.ia = (long long)ny * hx; gx.ib = hx; gx.ie = 1;
+  gx.oa = (long long)ny * nx; gx.ob = nx; gx.oe = 1;
+  gx.Lin = (int)hx; gx.Lout = nx;
+  if (fft_length_direct(nx)) {
+    AxisPlan px;
+    CL_TRY(axis_plan(nx, &px));
+    return fft_c2r_rows(spec, out, px, gx, s);
+  }
+  return fft_axis_large(spec, out, gx, nx, kGatherHalf, +1, nx, true, s);
 }
 
 ClarityResult_t fft3d_c2r_planned(const AxisPlan &px, const AxisPlan &py,

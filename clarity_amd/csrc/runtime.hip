@@ -172,6 +172,8 @@ ClarityResult_t Clarity_UnRegister() {
   if (r.stream) (void)hipStreamSynchronize(r.stream);
   for (auto &kv : r.twiddles) (void)hipFree(kv.second);
   r.twiddles.clear();
+  for (auto &kv : r.tables) (void)hipFree(kv.second);
+  r.tables.clear();
   dev_cache_clear();
   if (r.stream) (void)hipStreamDestroy(r.stream);
   r.stream = nullptr;
