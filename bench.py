@@ -270,10 +270,19 @@ def suite(api, threads):
     for a, k in zip(imgs[:3], ks[:3]):
         oracle.ml(a, k, 1)
         t_cpu += oracle.last_loop_seconds() * vox / sum(int(np.prod(x.shape)) for x in imgs[:3])
+    # bytes the volumes' kernels move per iteration (each plan's per-launch
+    # model x launches per iteration)
+    moved = 0.0
+    for a, k in zip(imgs, ks):
+        p = api.RLPlan(a.shape, k.shape)
+        moved += sum(b * n for _, b, n in p.describe())
+        p.close()
     line = {"suite": "batched rl 48 uneven volumes x 50 iterations",
             "gpu_ms": round(g * 1e3, 2), "Mvoxel_iters_per_s": round(vox * 50 / g / 1e6, 1),
             "setup_transfer_ms": round(g0 * 1e3, 2),
             "iterations_Mvoxel_iters_per_s": round(vox * 50 / max(g - g0, 1e-9) / 1e6, 1),
+            "moved_bytes_per_iteration": moved,
+            "iterations_moved_GBps": round(moved * 50 / max(g - g0, 1e-9) / 1e9, 1),
             "cpu_Mvoxel_iters_per_s": round(vox / t_cpu / 1e6, 2), "voxels": vox,
             "cpu_note": "1 iteration on 3 of the volumes, scaled by voxels",
             "cpu_fft": backend, "cores": threads}
