@@ -49,8 +49,8 @@ ClarityResult_t work_upload(Work &w, const float *in, const float *kernel,
   DevBuf img, ker;
   CL_TRY(img.alloc(sizeof(float) * w.IV));
   CL_TRY(ker.alloc(sizeof(float) * w.KV));
-  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(img.p, in, sizeof(float) * w.IV, s));
+  CL_TRY(host_to_device(ker.p, kernel, sizeof(float) * w.KV, s));
   CL_TRY(work_pad_device(w, img.f(), ker.f(), s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
@@ -68,7 +68,7 @@ ClarityResult_t work_download(const Work &w, const float *src, float *host_out,
   DevBuf o;
   CL_TRY(o.alloc(sizeof(float) * w.IV));
   CL_TRY(k_clip(o.f(), w.id, src, w.wd, s));
-  CL_HIP(hipMemcpyAsync(host_out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(host_out, o.p, sizeof(float) * w.IV, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -90,14 +90,14 @@ ClarityResult_t single_fused(const Work &w, const float *in, const float *kernel
   DevBuf img, ker;
   CL_TRY(img.alloc(sizeof(float) * w.IV));
   CL_TRY(ker.alloc(sizeof(float) * w.KV));
-  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(img.p, in, sizeof(float) * w.IV, s));
+  CL_TRY(host_to_device(ker.p, kernel, sizeof(float) * w.KV, s));
   CL_TRY(eng->allocate());
   CL_TRY(eng->set_inputs(img.f(), ker.f(), s));
   CL_TRY(eng->reset(nullptr, s));
   CL_TRY(eng->iterate(1, s));
   CL_TRY(eng->estimate(img.f(), s));
-  CL_HIP(hipMemcpyAsync(out, img.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(out, img.p, sizeof(float) * w.IV, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -156,14 +156,14 @@ ClarityResult_t jvc(float *in, Clarity_Dim3 id, float *kernel, Clarity_Dim3 kd,
   DevBuf img, ker, mparts, mval;
   CL_TRY(img.alloc(sizeof(float) * w.IV));
   CL_TRY(ker.alloc(sizeof(float) * w.KV));
-  CL_HIP(hipMemcpyAsync(img.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(ker.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(img.p, in, sizeof(float) * w.IV, s));
+  CL_TRY(host_to_device(ker.p, kernel, sizeof(float) * w.KV, s));
   CL_TRY(mparts.alloc(sizeof(float) * kParts));
   CL_TRY(mval.alloc(sizeof(float)));
   CL_TRY(k_max_partials(img.f(), w.IV, mparts.f(), kParts, s));
   CL_TRY(k_finish_max(mparts.f(), kParts, mval.f(), s));
   float mx = 0.f;
-  CL_HIP(hipMemcpyAsync(&mx, mval.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(&mx, mval.p, sizeof(float), s));
   CL_TRY(work_pad_device(w, img.f(), ker.f(), s));
   CL_HIP(hipStreamSynchronize(s));
   img.release();
@@ -225,8 +225,8 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
       CL_TRY(o.alloc(sizeof(float) * w.IV));
       CL_TRY(h.alloc(sizeof(float) * w.KV));
       CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
-      CL_HIP(hipMemcpyAsync(obs.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-      CL_HIP(hipMemcpyAsync(h.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+      CL_TRY(host_to_device(obs.p, in, sizeof(float) * w.IV, s));
+      CL_TRY(host_to_device(h.p, kernel, sizeof(float) * w.KV, s));
       CL_TRY(k_normalize_f32sum(h.f(), w.KV, parts.d(), kParts,
                                 reinterpret_cast<float *>(parts.d() + kParts), s));
       CL_TRY(eng->allocate());
@@ -234,7 +234,7 @@ ClarityResult_t blind_ml(float *in, Clarity_Dim3 id, float *kernel,
       CL_TRY(eng->reset(nullptr, s));
       CL_TRY(eng->iterate(iterations, s));
       CL_TRY(eng->estimate(o.f(), s));
-      CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+      CL_TRY(device_to_host(out, o.p, sizeof(float) * w.IV, s));
       CL_HIP(hipStreamSynchronize(s));
       return CLARITY_SUCCESS;
     }
@@ -285,10 +285,10 @@ ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kern
   CL_TRY(o.alloc(sizeof(float) * w.IV));
   CL_TRY(h.alloc(sizeof(float) * w.KV));
   CL_TRY(parts.alloc(sizeof(double) * (kParts + 2)));
-  CL_HIP(hipMemcpyAsync(obs.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(h.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(obs.p, in, sizeof(float) * w.IV, s));
+  CL_TRY(host_to_device(h.p, kernel, sizeof(float) * w.KV, s));
   if (initial)
-    CL_HIP(hipMemcpyAsync(o.p, initial, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+    CL_TRY(host_to_device(o.p, initial, sizeof(float) * w.IV, s));
   else
     CL_HIP(hipMemcpyAsync(o.p, obs.p, sizeof(float) * w.IV, hipMemcpyDeviceToDevice, s));
   CL_TRY(k_normalize_fp64(h.f(), w.KV, parts.d(), kParts, s));
@@ -303,8 +303,8 @@ ClarityResult_t blind_rl_fused(const Work &w, const float *in, const float *kern
     CL_TRY(ker->estimate(h.f(), s));
   }
   if (kernel_out)
-    CL_HIP(hipMemcpyAsync(kernel_out, h.p, sizeof(float) * w.KV, hipMemcpyDeviceToHost, s));
-  CL_HIP(hipMemcpyAsync(out, o.p, sizeof(float) * w.IV, hipMemcpyDeviceToHost, s));
+    CL_TRY(device_to_host(kernel_out, h.p, sizeof(float) * w.KV, s));
+  CL_TRY(device_to_host(out, o.p, sizeof(float) * w.IV, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -338,7 +338,7 @@ ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
   if (initial) {
     DevBuf init;
     CL_TRY(init.alloc(sizeof(float) * w.IV));
-    CL_HIP(hipMemcpyAsync(init.p, initial, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
+    CL_TRY(host_to_device(init.p, initial, sizeof(float) * w.IV, s));
     const int zero[3] = {0, 0, 0};
     CL_TRY(k_pad_shift(o.f(), w.wd, init.f(), w.id, zero, 0.f, s));
     CL_HIP(hipStreamSynchronize(s));
@@ -376,8 +376,7 @@ ClarityResult_t blind_rl(float *in, Clarity_Dim3 id, float *kernel,
     const int back[3] = {w.kd.x / 2, w.kd.y / 2, w.kd.z / 2};
     CL_TRY(k_pad_shift(un.f(), w.wd, h, w.wd, back, 0.f, s));
     CL_TRY(k_clip(ko.f(), w.kd, un.f(), w.wd, s));
-    CL_HIP(hipMemcpyAsync(kernel_out, ko.p, sizeof(float) * w.KV,
-                          hipMemcpyDeviceToHost, s));
+    CL_TRY(device_to_host(kernel_out, ko.p, sizeof(float) * w.KV, s));
   }
   return work_download(w, o.f(), out, s);
 }
@@ -391,9 +390,9 @@ ClarityResult_t fft_r2c_api(int nx, int ny, int nz, float *in, float *out) {
   DevBuf a, b;
   CL_TRY(a.alloc(sizeof(float) * V));
   CL_TRY(b.alloc(sizeof(float2) * C));
-  CL_HIP(hipMemcpyAsync(a.p, in, sizeof(float) * V, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, in, sizeof(float) * V, s));
   CL_TRY(fft3d_r2c(nx, ny, nz, a.f(), b.c(), s));
-  CL_HIP(hipMemcpyAsync(out, b.p, sizeof(float2) * C, hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(out, b.p, sizeof(float2) * C, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -407,9 +406,9 @@ ClarityResult_t fft_c2r_api(int nx, int ny, int nz, float *in, float *out) {
   DevBuf a, b;
   CL_TRY(a.alloc(sizeof(float2) * C));
   CL_TRY(b.alloc(sizeof(float) * V));
-  CL_HIP(hipMemcpyAsync(a.p, in, sizeof(float2) * C, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, in, sizeof(float2) * C, s));
   CL_TRY(fft3d_c2r(nx, ny, nz, a.c(), b.f(), s));
-  CL_HIP(hipMemcpyAsync(out, b.p, sizeof(float) * V, hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(out, b.p, sizeof(float) * V, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -425,9 +424,9 @@ ClarityResult_t pad_api(float *dst, Clarity_Dim3 dd, float *src,
   DevBuf a, b;
   CL_TRY(a.alloc(sizeof(float) * S.voxels()));
   CL_TRY(b.alloc(sizeof(float) * D.voxels()));
-  CL_HIP(hipMemcpyAsync(a.p, src, sizeof(float) * S.voxels(), hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, src, sizeof(float) * S.voxels(), s));
   CL_TRY(k_pad_shift(b.f(), D, a.f(), S, shift, fill, s));
-  CL_HIP(hipMemcpyAsync(dst, b.p, sizeof(float) * D.voxels(), hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(dst, b.p, sizeof(float) * D.voxels(), s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -443,9 +442,9 @@ ClarityResult_t clip_api(float *dst, Clarity_Dim3 dd, float *src,
   DevBuf a, b;
   CL_TRY(a.alloc(sizeof(float) * S.voxels()));
   CL_TRY(b.alloc(sizeof(float) * D.voxels()));
-  CL_HIP(hipMemcpyAsync(a.p, src, sizeof(float) * S.voxels(), hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, src, sizeof(float) * S.voxels(), s));
   CL_TRY(k_clip(b.f(), D, a.f(), S, s));
-  CL_HIP(hipMemcpyAsync(dst, b.p, sizeof(float) * D.voxels(), hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(dst, b.p, sizeof(float) * D.voxels(), s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }
@@ -459,12 +458,11 @@ ClarityResult_t idiv_api(const float *g, const float *ghat, int64_t n,
   CL_TRY(a.alloc(sizeof(float) * n));
   CL_TRY(b.alloc(sizeof(float) * n));
   CL_TRY(parts.alloc(sizeof(double) * kParts));
-  CL_HIP(hipMemcpyAsync(a.p, g, sizeof(float) * n, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(b.p, ghat, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, g, sizeof(float) * n, s));
+  CL_TRY(host_to_device(b.p, ghat, sizeof(float) * n, s));
   CL_TRY(k_idiv_partials(a.f(), b.f(), n, parts.d(), kParts, s));
   std::vector<double> h(kParts);
-  CL_HIP(hipMemcpyAsync(h.data(), parts.p, sizeof(double) * kParts,
-                        hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(h.data(), parts.p, sizeof(double) * kParts, s));
   CL_HIP(hipStreamSynchronize(s));
   double acc = 0.0;
   for (double v : h) acc += v;
@@ -483,10 +481,10 @@ ClarityResult_t divide_api(float *result, const float *a, const float *b,
   CL_TRY(da.alloc(sizeof(float) * n));
   CL_TRY(db.alloc(sizeof(float) * n));
   CL_TRY(dr.alloc(sizeof(float) * n));
-  CL_HIP(hipMemcpyAsync(da.p, a, sizeof(float) * n, hipMemcpyHostToDevice, s));
-  CL_HIP(hipMemcpyAsync(db.p, b, sizeof(float) * n, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(da.p, a, sizeof(float) * n, s));
+  CL_TRY(host_to_device(db.p, b, sizeof(float) * n, s));
   CL_TRY(k_guarded_divide(dr.f(), da.f(), db.f(), value, n, s));
-  CL_HIP(hipMemcpyAsync(result, dr.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(result, dr.p, sizeof(float) * n, s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }

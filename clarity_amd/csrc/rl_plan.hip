@@ -131,8 +131,8 @@ ClarityResult_t plan_set_inputs(ClarityEx_RLPlan p, const float *in,
     DevBuf a, b;
     CL_TRY(a.alloc(sizeof(float) * w.IV));
     CL_TRY(b.alloc(sizeof(float) * w.KV));
-    CL_HIP(hipMemcpyAsync(a.p, in, sizeof(float) * w.IV, hipMemcpyHostToDevice, s));
-    CL_HIP(hipMemcpyAsync(b.p, kernel, sizeof(float) * w.KV, hipMemcpyHostToDevice, s));
+    CL_TRY(host_to_device(a.p, in, sizeof(float) * w.IV, s));
+    CL_TRY(host_to_device(b.p, kernel, sizeof(float) * w.KV, s));
     CL_TRY(p->eng->set_inputs(a.f(), b.f(), s));
     CL_HIP(hipStreamSynchronize(s));
   }
@@ -147,7 +147,7 @@ ClarityResult_t plan_reset(ClarityEx_RLPlan p, const float *init, int on_device)
   if (!init || on_device) return p->eng->reset(init, s);
   DevBuf a;
   CL_TRY(a.alloc(sizeof(float) * p->geom.IV));
-  CL_HIP(hipMemcpyAsync(a.p, init, sizeof(float) * p->geom.IV, hipMemcpyHostToDevice, s));
+  CL_TRY(host_to_device(a.p, init, sizeof(float) * p->geom.IV, s));
   CL_TRY(p->eng->reset(a.f(), s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
@@ -163,7 +163,7 @@ ClarityResult_t plan_get(ClarityEx_RLPlan p, float *out, int on_device) {
     DevBuf a;
     CL_TRY(a.alloc(sizeof(float) * p->geom.IV));
     CL_TRY(p->eng->estimate(a.f(), s));
-    CL_HIP(hipMemcpyAsync(out, a.p, sizeof(float) * p->geom.IV, hipMemcpyDeviceToHost, s));
+    CL_TRY(device_to_host(out, a.p, sizeof(float) * p->geom.IV, s));
   }
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
@@ -218,11 +218,9 @@ ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
     rc = vols[v].a.alloc(sizeof(float) * w.IV);
     if (rc == CLARITY_SUCCESS) rc = vols[v].b.alloc(sizeof(float) * w.KV);
     if (rc == CLARITY_SUCCESS)
-      rc = from_hip(hipMemcpyAsync(vols[v].a.p, ins[v], sizeof(float) * w.IV,
-                                   hipMemcpyHostToDevice, s));
+      rc = host_to_device(vols[v].a.p, ins[v], sizeof(float) * w.IV, s);
     if (rc == CLARITY_SUCCESS)
-      rc = from_hip(hipMemcpyAsync(vols[v].b.p, kers[v], sizeof(float) * w.KV,
-                                   hipMemcpyHostToDevice, s));
+      rc = host_to_device(vols[v].b.p, kers[v], sizeof(float) * w.KV, s);
     if (rc == CLARITY_SUCCESS) rc = p->eng->set_inputs(vols[v].a.f(), vols[v].b.f(), s);
     if (rc == CLARITY_SUCCESS) rc = p->eng->reset(nullptr, s);
     p->have_inputs = true;
@@ -257,8 +255,7 @@ ClarityResult_t ml_batch(int count, float *const *ins, const Clarity_Dim3 *ids,
     const Work &w = vols[v].plan->geom;
     rc = vols[v].plan->eng->estimate(vols[v].a.f(), s);
     if (rc == CLARITY_SUCCESS)
-      rc = from_hip(hipMemcpyAsync(outs[v], vols[v].a.p, sizeof(float) * w.IV,
-                                   hipMemcpyDeviceToHost, s));
+      rc = device_to_host(outs[v], vols[v].a.p, sizeof(float) * w.IV, s);
   }
   for (hipStream_t s : streams) {
     if (!s) continue;
@@ -342,8 +339,7 @@ ClarityResult_t ClarityEx_RLPlanGetScale(ClarityEx_RLPlan plan, float *scale) {
   API_LOCK;
   if (!plan || !scale) return CLARITY_INVALID_ARGUMENT;
   hipStream_t s = rt().stream;
-  CL_HIP(hipMemcpyAsync(scale, plan->eng->scale_ptr(), sizeof(float),
-                        hipMemcpyDeviceToHost, s));
+  CL_TRY(device_to_host(scale, plan->eng->scale_ptr(), sizeof(float), s));
   CL_HIP(hipStreamSynchronize(s));
   return CLARITY_SUCCESS;
 }

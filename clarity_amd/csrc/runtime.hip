@@ -2,9 +2,12 @@
 // selection, the library stream and the device-memory cache.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 namespace clarity {
 
@@ -76,6 +79,101 @@ ClarityResult_t dev_alloc(void **p, size_t bytes) {
       *p = nullptr;
       return CLARITY_DEVICE_OUT_OF_MEMORY;
     }
+  }
+  return CLARITY_SUCCESS;
+}
+
+namespace {
+
+constexpr size_t kStageBytes = (size_t)32 << 20;  // per staging buffer
+constexpr size_t kStageMin = (size_t)4 << 20;     // smaller copies go directly
+
+bool stage_ready(Runtime &r) {
+  if (r.stage[0] && r.stage[1]) return true;
+  for (int i = 0; i < 2; i++) {
+    if (!r.stage[i] && hipHostMalloc(&r.stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      r.stage[i] = nullptr;
+      return false;
+    }
+    if (!r.stage_ev[i] &&
+        hipEventCreateWithFlags(&r.stage_ev[i], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      r.stage_ev[i] = nullptr;
+      return false;
+    }
+  }
+  return true;
+}
+
+// memcpy split over up to 8 threads (64-byte aligned pieces)
+void par_copy(void *dst, const void *src, size_t n) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int T = (int)std::min<unsigned>(8u, hw ? hw : 1u);
+  if (T <= 1 || n < ((size_t)1 << 20)) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t per = ((n + T - 1) / T + 63) & ~(size_t)63;
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; t++) {
+    const size_t o = (size_t)t * per;
+    if (o >= n) break;
+    const size_t len = std::min(per, n - o);
+    th.emplace_back([=] {
+      std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, len);
+    });
+  }
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+ClarityResult_t host_to_device(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  Runtime &r = rt();
+  if (bytes < kStageMin || !stage_ready(r))
+    return from_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  for (size_t o = 0; o < bytes; o += kStageBytes) {
+    const int b = r.stage_next;
+    r.stage_next ^= 1;
+    const size_t len = std::min(kStageBytes, bytes - o);
+    CL_HIP(hipEventSynchronize(r.stage_ev[b]));  // its previous DMA is done
+    par_copy(r.stage[b], static_cast<const char *>(src) + o, len);
+    CL_HIP(hipMemcpyAsync(static_cast<char *>(dst) + o, r.stage[b], len,
+                          hipMemcpyHostToDevice, s));
+    CL_HIP(hipEventRecord(r.stage_ev[b], s));
+  }
+  return CLARITY_SUCCESS;
+}
+
+ClarityResult_t device_to_host(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  Runtime &r = rt();
+  if (bytes < kStageMin || !stage_ready(r)) {
+    CL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    CL_HIP(hipStreamSynchronize(s));
+    return CLARITY_SUCCESS;
+  }
+  const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+  const int base = r.stage_next;
+  r.stage_next = (int)((base + nch) & 1);
+  // chunk i goes through buffer (base + i) % 2; issuing it waits for that
+  // buffer's previous DMA (chunk i - 2's host copy has finished already)
+  const auto issue = [&](size_t i) -> ClarityResult_t {
+    const int b = (int)((base + i) & 1);
+    const size_t o = i * kStageBytes, len = std::min(kStageBytes, bytes - o);
+    CL_HIP(hipEventSynchronize(r.stage_ev[b]));
+    CL_HIP(hipMemcpyAsync(r.stage[b], static_cast<const char *>(src) + o, len,
+                          hipMemcpyDeviceToHost, s));
+    CL_HIP(hipEventRecord(r.stage_ev[b], s));
+    return CLARITY_SUCCESS;
+  };
+  CL_TRY(issue(0));
+  for (size_t i = 0; i < nch; i++) {
+    if (i + 1 < nch) CL_TRY(issue(i + 1));
+    const int b = (int)((base + i) & 1);
+    const size_t o = i * kStageBytes, len = std::min(kStageBytes, bytes - o);
+    CL_HIP(hipEventSynchronize(r.stage_ev[b]));
+    par_copy(static_cast<char *>(dst) + o, r.stage[b], len);
   }
   return CLARITY_SUCCESS;
 }
@@ -174,6 +272,13 @@ ClarityResult_t Clarity_UnRegister() {
   r.twiddles.clear();
   for (auto &kv : r.tables) (void)hipFree(kv.second);
   r.tables.clear();
+  for (int i = 0; i < 2; i++) {
+    if (r.stage[i]) (void)hipHostFree(r.stage[i]);
+    if (r.stage_ev[i]) (void)hipEventDestroy(r.stage_ev[i]);
+    r.stage[i] = nullptr;
+    r.stage_ev[i] = nullptr;
+  }
+  r.stage_next = 0;
   dev_cache_clear();
   if (r.stream) (void)hipStreamDestroy(r.stream);
   r.stream = nullptr;
