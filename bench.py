@@ -436,6 +436,18 @@ def main():
                "fft": backend}
         log(f"[bench] cpu baseline {cpu['value']} Mvoxel-iters/s; {rate}")
 
+    # the whole C-API call with host buffers (upload, pad, kernel transform,
+    # iterations, clip, download: the reference's TIME_TOTAL scope), beside
+    # `value`'s resident deconvolution
+    api.MaximumLikelihoodDeconvolve(obs, psf, args.iterations)
+    tc = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        api.MaximumLikelihoodDeconvolve(obs, psf, args.iterations)
+        tc.append(time.perf_counter() - t1)
+    c_api_ms = 1e3 * float(np.median(tc))
+    log(f"[bench] C-API call with host buffers: {c_api_ms:.1f} ms")
+
     # time-to-accuracy of the learned warm start against plain RL, on the
     # same volume (network + 20 RL iterations vs plain RL, measured against
     # plain RL after 200 iterations)
@@ -501,7 +513,10 @@ def main():
                    "pruned_model_frac": round((0.5 * R + 8 * C) / it_s / 1e9 / HBM_PEAK_GBS, 4),
                    "moved_bytes_per_iteration": moved,
                    "moved_GBps": round(moved / it_s / 1e9, 1) if moved else None,
-                   "moved_frac": round(moved / it_s / 1e9 / HBM_PEAK_GBS, 4) if moved else None},
+                   "moved_frac": round(moved / it_s / 1e9 / HBM_PEAK_GBS, 4) if moved else None,
+                   "c_api_call_ms": round(c_api_ms, 2),
+                   "c_api_Mvoxel_iters_per_s": round(vox * args.iterations / c_api_ms * 1e3 / 1e6,
+                                                     1)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned_warm_start": learned,

@@ -4,7 +4,8 @@ The fused RL engine at 512x512x128 (100 iterations, same-size side-lobe
 kernel) against the oracle on the FFTW3 float API (the reference's FFT
 library interface; MKL's implementation where libfftw3f is absent) with
 float64 reductions, the library's rule; the deviation from the reference's
-literal sequential float32 energy sum is printed beside it.  At
+literal sequential float32 energy sum is printed beside it; 1 and 20
+iterations against the oracle's float64 FFT.  At
 256x256x64: RL 100 iterations, Wiener (epsilon 1e-3) and Jansen-van Cittert
 (50 iterations) against the oracle's float64 FFT, so that the error against
 a float64 computation is stated.  Blind RL at 512x512x128 on its 20 x 5
@@ -84,6 +85,24 @@ def test_rl_512_100_iterations_vs_oracle(clarity_gpu):
           f"{time.perf_counter() - t0:.1f} s; rel L2 GPU vs oracle {err:.3e}", flush=True)
     assert err <= 1e-4
     assert np.all(got >= 0)
+
+
+@pytest.mark.parametrize("iters,tol", [(1, 1e-5), (20, 1e-4)])
+def test_rl_512_vs_fp64_fft_oracle(clarity_gpu, iters, tol):
+    """Fused RL on the headline workload against the oracle's float64 FFT
+    with float64 sums (the error against a float64 computation at full
+    size): <= 1e-5 after one iteration, <= 1e-4 after 20 (the float64 FFT
+    takes ~3.5 s per iteration here, so 100 would not fit a test)."""
+    api = clarity_gpu
+    case = _workload(api, "filaments-512x512x128")
+    got = api.MaximumLikelihoodDeconvolve(case.obs, case.psf, iters)
+    assert api.LastEngine() == "fast"
+    t0 = time.perf_counter()
+    ref = oracle.ml(case.obs, case.psf, iters)
+    err = rel_l2(got, ref)
+    print(f"[fullsize] RL {iters} it 512x512x128: fp64-FFT oracle "
+          f"{time.perf_counter() - t0:.1f} s; rel L2 {err:.3e}", flush=True)
+    assert err <= tol
 
 
 def test_rl_512_100_iterations_serial_f32_deviation(clarity_gpu):
