@@ -637,6 +637,33 @@ template <> struct Plan<192, 12> { using Rad = Radices<12, 4, 4>; };
 template <> struct Plan<384, 12> { using Rad = Radices<6, 4, 4, 4>; };
 template <> struct Plan<768, 12> { using Rad = Radices<12, 4, 4, 4>; };
 
+// Entries of an N-point twiddle table that fft_pruned reads for a radix
+// list: a stage of radix R at span P reads entry k * N / (P R) (R = 3, 4:
+// one read, powers by products) or r k N / (P R), r < R (other radices),
+// for k < P.  A kernel stages only these in LDS (a 768-point y pencil with
+// radices 12, 4, 4, 4 reads the first 192 of 768).
+template <int N, int P, class Rad>
+struct TwNeed;
+template <int N, int P>
+struct TwNeed<N, P, Radices<>> {
+  static constexpr int value = 1;
+};
+template <int N, int P, int R, int... Rest>
+struct TwNeed<N, P, Radices<R, Rest...>> {
+  static constexpr int here =
+      P == 1 ? 1 : ((R == 3 || R == 4) ? (P - 1) : (R - 1) * (P - 1)) * (N / (P * R)) + 1;
+  static constexpr int rest = TwNeed<N, P * R, Radices<Rest...>>::value;
+  static constexpr int value = here > rest ? here : rest;
+};
+template <int N, int E>
+constexpr int kTwNeed = TwNeed<N, 1, typename Plan<N, E>::Rad>::value;
+// the largest index read plus one, from the stage loop run out by hand
+static_assert(kTwNeed<768, 12> == 192 && kTwNeed<384, 12> == 96 && kTwNeed<192, 12> == 48 &&
+                  kTwNeed<96, 12> == 48 && kTwNeed<1024, 16> == 901 &&
+                  kTwNeed<512, 16> == 451 && kTwNeed<256, 16> == 226 &&
+                  kTwNeed<128, 16> == 106 && kTwNeed<2048, 16> == 1801,
+              "twiddle entries read");
+
 struct Geom {
   int ix, iy, iz;  // image
   int nx, ny, nz;  // work
@@ -1322,7 +1349,10 @@ __device__ __forceinline__ void y_body(int blk, float2 *__restrict__ S, const fl
   constexpr int PS = 30;
   constexpr int NP = kPadded<N, PS>;
   __shared__ float2 lds[PPW * NP];
-  __shared__ float2 tw[N];
+  // only the entries the stages read: at 768 points the LDS of a workgroup
+  // then allows six per CU instead of five
+  constexpr int TWN = kTwNeed<N, E>;
+  __shared__ float2 tw[TWN];
   const int pl = threadIdx.x / T, t = threadIdx.x % T;
   const long long p = (long long)blk * PPW + pl;
   const bool valid = p < npencils;
@@ -1342,7 +1372,7 @@ __device__ __forceinline__ void y_body(int blk, float2 *__restrict__ S, const fl
       v[e] = (j < Lin) ? x : make_float2(0.f, 0.f);
     }
   }
-  stage_table<NT, N>(tw, gtw);
+  stage_table<NT, TWN>(tw, gtw);
   __syncthreads();
   const PencilLds<(T <= 64), PS> L{lds + pl * NP};
   fft_stages<N, E, 1, SIGN>(v, t, L, tw, typename Plan<N, E>::Rad{});
