@@ -247,20 +247,8 @@ def suite(api, threads):
     emit("blind rl 20 outer x 5 inner 512x512x128", g, 100 * cb, 0,
          "cpu: 100 x the cost of a 1 x 1 call")
     del obj_b, psf_b, obs_b, k0_b
-    # 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels; the
-    # shapes drawn first from their own seed, so they do not depend on the
-    # values (and random draws) of the contents
-    srng = np.random.default_rng(4)
-    shapes = [(int(srng.choice([32, 48, 64, 96])), int(srng.choice([64, 96, 128, 192, 256])),
-               int(srng.choice([64, 96, 128, 192, 256]))) for _ in range(48)]
-    rng = np.random.default_rng(5)
-    imgs, ks = [], []
-    for s in shapes:
-        f = np.prod(s) / np.prod(shape)
-        o = synth.beads_and_filaments(s, int(2000 * f) + 1, int(300 * f) + 1, rng)
-        k = synth.gaussian_kernel(s, rng.uniform(1.2, 2.5), rng.uniform(2.5, 5.0))
-        imgs.append(synth.poisson(api.Convolve(o, k), rng))
-        ks.append(k)
+    # 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
+    imgs, ks = synth.make_batch(convolve=api.Convolve)
     vox = sum(int(np.prod(a.shape)) for a in imgs)
     g = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 50), reps=1)
     # the same call with no iterations: plans, uploads, kernel transforms,

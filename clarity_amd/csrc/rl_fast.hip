@@ -718,6 +718,27 @@ struct TableRegs {
   }
 };
 
+// Sum of the workgroup's NT per-thread values in a fixed order: a butterfly
+// across each wavefront's 64 lanes (cross-lane moves, no barrier), one LDS
+// slot per wavefront, one barrier, and thread 0 adds the wavefronts' sums in
+// order.  `sh` holds NT/64 doubles and must be free for the workgroup to
+// overwrite (the caller synchronises before reusing it).  Returns the sum
+// in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double acc, double *sh) {
+  static_assert(NT % 64 == 0, "whole wavefronts");
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) acc += __shfl_xor(acc, m, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) s += sh[w];
+  }
+  return s;
+}
+
 template <int NT, int N>
 __device__ __forceinline__ void stage_table(float2 *dst, const float2 *__restrict__ src) {
   TableRegs<NT, N> r;
@@ -1187,15 +1208,10 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
       if (second) acc += (double)u1;
     }
   }
-  double *sh = reinterpret_cast<double *>(sm.tile);
+  // every wavefront is done with the tile before its first slots are reused
   __syncthreads();
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partials[tile] = sh[0];
+  const double sum = block_sum<NT>(acc, reinterpret_cast<double *>(sm.tile));
+  if (threadIdx.x == 0) partials[tile] = sum;
 }
 
 template <int M, int E, int TP, int HE>
@@ -1283,15 +1299,10 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   __syncthreads();
   unpack_store<M, TP, NT, PS>(sm.tile, S, sm.twN, g, z, y0);
   if constexpr (MODE == 1) return;
-  double *sh = reinterpret_cast<double *>(sm.tile);
+  // every wavefront is done with the tile before its first slots are reused
   __syncthreads();
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partials[tile] = sh[0];
+  const double sum = block_sum<NT>(acc, reinterpret_cast<double *>(sm.tile));
+  if (threadIdx.x == 0) partials[tile] = sum;
 }
 
 template <int M, int E, int TP, int HE>

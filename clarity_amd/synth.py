@@ -125,6 +125,28 @@ def make_workload(name: str, seed: int | None = None, convolve=np_convolve,
     return Case(obj, psf, obs)
 
 
+def make_batch(convolve=np_convolve, count: int = 48):
+    """The uneven-volume batch: ``count`` volumes (z, y, x) with x, y drawn
+    from {64, 96, 128, 192, 256} and z from {32, 48, 64, 96} (shapes from
+    seed 4, drawn before and independently of the contents), each with
+    the 256x256x64 filament statistics scaled by its volume and its own
+    Gaussian kernel of its size, sigma_xy ~ U[1.2, 2.5], sigma_z ~
+    U[2.5, 5.0] (contents from seed 5).  Returns (observations, kernels)."""
+    srng = np.random.default_rng(4)
+    shapes = [(int(srng.choice([32, 48, 64, 96])), int(srng.choice([64, 96, 128, 192, 256])),
+               int(srng.choice([64, 96, 128, 192, 256]))) for _ in range(count)]
+    rng = np.random.default_rng(5)
+    base = (64, 256, 256)
+    imgs, ks = [], []
+    for s in shapes:
+        f = np.prod(s) / np.prod(base)
+        o = beads_and_filaments(s, int(2000 * f) + 1, int(300 * f) + 1, rng)
+        k = gaussian_kernel(s, rng.uniform(1.2, 2.5), rng.uniform(2.5, 5.0))
+        imgs.append(poisson(convolve(o, k), rng))
+        ks.append(k)
+    return imgs, ks
+
+
 def blind_initial_kernel(name: str, shape) -> np.ndarray:
     """Blind-ML starting kernel: the workload's Gaussian 20 % too wide."""
     if name == "beads-64x64x32":

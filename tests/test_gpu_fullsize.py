@@ -213,3 +213,36 @@ def test_blind_rl_512_20x5(clarity_gpu):
                - 1.0) < 1e-4
     assert e1 < e0
     assert so <= 1e-3 and sh <= 1e-3
+
+
+def test_batch_48_uneven_volumes_50_iterations(clarity_gpu):
+    """Batched RL on its benchmark draw: 48 uneven volumes (x, y in
+    64..256, z in 32..96, each with its own Gaussian kernel), 50
+    iterations, one grouped call.  Every volume is bit for bit its own
+    single-volume call; the three largest volumes against the oracle on the
+    FFTW3 float API with float64 sums and the three smallest against the
+    oracle's float64 FFT, rel L2 <= 1e-4."""
+    api = clarity_gpu
+    t0 = time.perf_counter()
+    imgs, ks = synth.make_batch(convolve=api.Convolve)
+    print(f"[fullsize] batch inputs in {time.perf_counter() - t0:.1f} s", flush=True)
+    t0 = time.perf_counter()
+    outs = api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 50)
+    print(f"[fullsize] batch 48 x 50 it: {time.perf_counter() - t0:.2f} s (C call)", flush=True)
+    for i, (a, k, o) in enumerate(zip(imgs, ks, outs)):
+        single = api.MaximumLikelihoodDeconvolve(a, k, 50)
+        assert api.LastEngine() == "fast", i
+        assert np.array_equal(o, single), i
+    order = sorted(range(len(imgs)), key=lambda i: imgs[i].size)
+    checks = [(i, oracle.FFT_OWN) for i in order[:3]]
+    if oracle.set_fft_backend(oracle.FFT_FFTW_API):
+        checks += [(i, oracle.FFT_FFTW_API) for i in order[-3:]]
+    for i, backend in checks:
+        oracle.set_fft_backend(backend)
+        t0 = time.perf_counter()
+        ref = oracle.ml(imgs[i], ks[i], 50)
+        err = rel_l2(outs[i], ref)
+        print(f"[fullsize] batch volume {i} {imgs[i].shape}: oracle "
+              f"({oracle.fft_backend_name()}) {time.perf_counter() - t0:.1f} s; "
+              f"rel L2 {err:.3e}", flush=True)
+        assert err <= 1e-4, i
