@@ -673,6 +673,8 @@ struct Geom {
   // and z >= nz - (kz - ka), stored in K as planes p = z and
   // p = z - (nz - kz) (ka = kz - kz/2)
   int kz, ka;
+  // x passes take their tiles from the last plane down (1) instead of up
+  int xrev;
 };
 
 // An engine's device buffers and twiddle tables.
@@ -685,13 +687,80 @@ struct Bufs {
 
 // One volume of a batched launch: a kernel instantiation run over several
 // volumes at once reads its volume's geometry and buffers here (the entry
-// of vmap[blockIdx.x]) and works on block blockIdx.x - blk0 of it.
+// volume_of finds for blockIdx.x) and works on block blockIdx.x - blk0 of
+// it.  The
+// buffers' addresses in b are byte offsets from the table's own device
+// address (relative_to, bufs_at): a pointer read from memory is generic to
+// the compiler, which then addresses it with flat instructions (counted
+// together with LDS operations, so every wait for an LDS access also waits
+// for the memory accesses in flight); an address formed from the table's
+// (a kernel argument, known to be global memory) gets global instructions.
 struct VolTab {
   Geom g;
   Bufs b;
   int blk0;    // the volume's first block in the launch
   int nparts;  // finish: partial sums of the volume's last x pass
 };
+
+// The volume whose blocks include block b: the last i < n with blk0s[i] <= b
+// (blk0s ascending, blk0s[0] = 0).  A binary search over the launch's short
+// list of first blocks, which every workgroup reads (so it stays in the
+// scalar cache), instead of a per-block map, whose entry for each new
+// workgroup is a cache miss on the way to its first loads.
+__device__ __forceinline__ int volume_of(const int *__restrict__ blk0s, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0s[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T>
+T *offset_from(T *p, const void *base) {
+  return reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) -
+                               reinterpret_cast<uintptr_t>(base));
+}
+// The table entry t with its buffer addresses made relative to base, the
+// device address of the table it is copied to.
+VolTab relative_to(VolTab t, const void *base) {
+  Bufs &b = t.b;
+  b.u = offset_from(b.u, base);
+  b.obs = offset_from(b.obs, base);
+  b.scal = offset_from(b.scal, base);
+  b.S = offset_from(b.S, base);
+  b.K = offset_from(b.K, base);
+  b.parts = offset_from(b.parts, base);
+  b.twM = offset_from(b.twM, base);
+  b.twN = offset_from(b.twN, base);
+  b.twY = offset_from(b.twY, base);
+  b.twZ = offset_from(b.twZ, base);
+  return t;
+}
+template <class T>
+__device__ __forceinline__ T *at_offset(const void *base, T *off) {
+  return reinterpret_cast<T *>(const_cast<char *>(static_cast<const char *>(base)) +
+                               reinterpret_cast<intptr_t>(off));
+}
+// The buffers of a table entry, from the table's address (the kernel's
+// table argument).
+__device__ __forceinline__ Bufs bufs_at(const VolTab *vt, const Bufs &o) {
+  Bufs b;
+  b.u = at_offset(vt, o.u);
+  b.obs = at_offset(vt, o.obs);
+  b.scal = at_offset(vt, o.scal);
+  b.S = at_offset(vt, o.S);
+  b.K = at_offset(vt, o.K);
+  b.parts = at_offset(vt, o.parts);
+  b.twM = at_offset(vt, o.twM);
+  b.twN = at_offset(vt, o.twN);
+  b.twY = at_offset(vt, o.twY);
+  b.twZ = at_offset(vt, o.twZ);
+  return b;
+}
 
 // Copies a twiddle table into LDS (cooperatively; caller synchronises).
 // A table of N entries moved from global memory to LDS by NT threads in two
@@ -973,15 +1042,19 @@ __device__ __forceinline__ void store_row_pair(float *row, int n0, int ix, float
 // dispatcher sends to the same XCD (round-robin over 8) at nearly the same
 // time, take the two halves, so the line is fetched into that XCD's L2 once.
 // The grid is padded to a multiple of 16; padding blocks get -1.
+// With rev the tiles are taken in the opposite order (pairs of 8-row tiles
+// stay pairs: tiles 2j, 2j + 1 become ntiles - 1 - 2j, ntiles - 2 - 2j).
 template <int TP>
-__device__ __forceinline__ int x_tile(int b, int ntiles) {
+__device__ __forceinline__ int x_tile(int b, int ntiles, int rev) {
+  int t;
   if constexpr (TP == 16) {
-    return b < ntiles ? b : -1;
+    t = b;
   } else {
     const int loc = b & 15;
-    const int t = (b - loc) + (loc < 8 ? 2 * loc : 2 * (loc - 8) + 1);
-    return t < ntiles ? t : -1;
+    t = (b - loc) + (loc < 8 ? 2 * loc : 2 * (loc - 8) + 1);
   }
+  if (t >= ntiles) return -1;
+  return rev ? ntiles - 1 - t : t;
 }
 
 template <int M, int TP, int PS>
@@ -1000,7 +1073,7 @@ __device__ __forceinline__ void x_fwd_body(int blk, const float *__restrict__ u,
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(blk, g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile, g.xrev);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -1030,9 +1103,10 @@ __global__ void __launch_bounds__(TP *(M / E))
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    kb_x_fwd(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  x_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.u, v.b.S, v.b.twM, v.b.twN, v.g);
+    kb_x_fwd(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  const VolTab &v = vt[volume_of(blk0s, nvol, (int)blockIdx.x)];
+  const Bufs b = bufs_at(vt, v.b);
+  x_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, b.u, b.S, b.twM, b.twN, v.g);
 }
 
 template <int M, int E, int TP, int HE>
@@ -1045,7 +1119,7 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(blk, g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile, g.xrev);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -1108,9 +1182,10 @@ __global__ void __launch_bounds__(TP *(M / E))
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    kb_x_mid(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  x_mid_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.obs, v.b.scal + 2, v.b.twM, v.b.twN, v.g);
+    kb_x_mid(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  const VolTab &v = vt[volume_of(blk0s, nvol, (int)blockIdx.x)];
+  const Bufs b = bufs_at(vt, v.b);
+  x_mid_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, b.S, b.obs, b.scal + 2, b.twM, b.twN, v.g);
 }
 
 // Jansen-van Cittert update (reference src/JansenVanCittertDeconvolve.cxx:
@@ -1141,7 +1216,7 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(blk, g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile, g.xrev);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -1223,9 +1298,10 @@ __global__ void __launch_bounds__(TP *(M / E))
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    kb_x_last(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  x_last_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, nullptr, v.g);
+    kb_x_last(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  const VolTab &v = vt[volume_of(blk0s, nvol, (int)blockIdx.x)];
+  const Bufs b = bufs_at(vt, v.b);
+  x_last_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, b.S, b.u, b.scal + 2, b.parts, b.twM, b.twN, nullptr, v.g);
 }
 
 // x_last of one iteration fused with x_fwd of the next: the updated rows
@@ -1244,7 +1320,7 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   __shared__ XSmem<M, TP, PS> sm;
   const int q = threadIdx.x / T, t = threadIdx.x % T;
   const int ntile = (g.iy + TP - 1) / TP;
-  const int tile = x_tile<TP>(blk, g.iz * ntile);
+  const int tile = x_tile<TP>(blk, g.iz * ntile, g.xrev);
   if (tile < 0) return;  // whole workgroup: padding block of the remap
   const int z = tile / ntile;
   const int y0 = (tile % ntile) * TP;
@@ -1314,9 +1390,10 @@ __global__ void __launch_bounds__(TP *(M / E))
 }
 template <int M, int E, int TP, int HE>
 __global__ void __launch_bounds__(TP *(M / E))
-    kb_x_last_fwd(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  x_last_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, v.b.S, v.b.u, v.b.scal + 2, v.b.parts, v.b.twM, v.b.twN, nullptr, v.g);
+    kb_x_last_fwd(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  const VolTab &v = vt[volume_of(blk0s, nvol, (int)blockIdx.x)];
+  const Bufs b = bufs_at(vt, v.b);
+  x_last_fwd_body<M, E, TP, HE>((int)blockIdx.x - v.blk0, b.S, b.u, b.scal + 2, b.parts, b.twM, b.twN, nullptr, v.g);
 }
 
 // The last x pass of a Jansen-van Cittert iteration (C2R, update) and the
@@ -1406,9 +1483,15 @@ __global__ void __launch_bounds__(PPW *(N / E))
 }
 template <int N, int E, int PPW, int SIGN, bool H>
 __global__ void __launch_bounds__(PPW *(N / E))
-    kb_y(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  y_body<N, E, PPW, SIGN, H>((int)blockIdx.x - v.blk0, v.b.S, v.b.twY, (long long)v.g.iz * v.g.hx, SIGN < 0 ? v.g.iy : N, SIGN < 0 ? N : v.g.iy);
+    kb_y(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  // the grid walked backwards: the y passes take the volumes (and each
+  // volume's pencils) in the opposite order of the x and z passes before
+  // them, so they start on the data those wrote last, the likeliest to be
+  // still cached
+  const int blk = (int)(gridDim.x - 1 - blockIdx.x);
+  const VolTab &v = vt[volume_of(blk0s, nvol, blk)];
+  const Bufs b = bufs_at(vt, v.b);
+  y_body<N, E, PPW, SIGN, H>(blk - v.blk0, b.S, b.twY, (long long)v.g.iz * v.g.hx, SIGN < 0 ? v.g.iy : N, SIGN < 0 ? N : v.g.iy);
 }
 
 // A workgroup transforms TP adjacent ky columns of one kx (TP * 8 bytes
@@ -1537,9 +1620,10 @@ __global__ void __launch_bounds__(TP *(N / E))
 // batched: RL volumes only (make_rl_group takes no kernel-step engine)
 template <int N, int E, int TP, int HZ>
 __global__ void __launch_bounds__(TP *(N / E))
-    kb_z(const VolTab *__restrict__ vt, const unsigned *__restrict__ vmap) {
-  const VolTab &v = vt[vmap[blockIdx.x]];
-  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, v.b.S, v.b.K, v.b.twZ, v.g);
+    kb_z(const VolTab *__restrict__ vt, const int *__restrict__ blk0s, int nvol) {
+  const VolTab &v = vt[volume_of(blk0s, nvol, (int)blockIdx.x)];
+  const Bufs b = bufs_at(vt, v.b);
+  z_body<N, E, TP, HZ>((int)blockIdx.x - v.blk0, b.S, b.K, b.twZ, v.g);
 }
 // z pass of the Wiener filter (single volume)
 template <int N, int E, int TP, int HZ>
@@ -1669,6 +1753,7 @@ struct Tune {
   int yp = 0, ze = 0, zt = 16;
   int zw = 1;                // same-size z pass: column tiles per workgroup
   int fx = 1;                // chain x_last into the next x_fwd
+  int xr = 1;                // x passes walk their tiles from the last plane down
   int xh = 7;                // bit j: same-size specialisation of x kernel j
                              // (x_fwd, x_mid, x_last, x_last_fwd); measured
                              // slower for x_last_fwd with 8-row tiles and 8
@@ -1993,6 +2078,7 @@ Tune pick_tune(const Geom &g) {
       else if (std::sscanf(p, "zw=%d", &v) == 1) t.zw = v;
       else if (std::sscanf(p, "fx=%d", &v) == 1) t.fx = v;
       else if (std::sscanf(p, "xh=%d", &v) == 1) t.xh = v;
+      else if (std::sscanf(p, "xr=%d", &v) == 1) t.xr = v;
       while (*p && *p != ',') p++;
       if (*p == ',') p++;
     }
@@ -2028,6 +2114,7 @@ class FastRL : public RLEngine {
     auto hit = tune_cache().find(tune_key(g_));
     if (hit != tune_cache().end() && !std::getenv("CLARITY_FAST_TUNE")) tu_ = hit->second;
     nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
+    g_.xrev = tu_.xr;
   }
   ~FastRL() override {
     for (auto &ge : graph_exec_)
@@ -2056,7 +2143,12 @@ class FastRL : public RLEngine {
     g_rec = nullptr;
     return r;
   }
-  VolTab table_entry() const { return VolTab{g_, b_, 0, nparts_}; }
+  // (x passes in plane order: a batch reverses its y passes instead)
+  VolTab table_entry() const {
+    VolTab t{g_, b_, 0, nparts_};
+    t.g.xrev = 0;
+    return t;
+  }
 
   // The kernel half-step of alternating blind RL: the correction a
   // correlation (the blurring input's conjugate spectrum), the estimate
@@ -2354,6 +2446,7 @@ class FastRL : public RLEngine {
   void apply_tune(const Tune &t) {
     tu_ = t;
     nparts_ = x_tiles(g_, tu_.xt[2] == 8 ? 8 : 16);
+    g_.xrev = tu_.xr;
     for (auto &ge : graph_exec_) {
       if (ge) (void)hipGraphExecDestroy(ge);
       ge = nullptr;
@@ -2512,8 +2605,9 @@ class FastRL : public RLEngine {
 __global__ void __launch_bounds__(256) kb_finish(const VolTab *__restrict__ vt) {
   __shared__ double sh[256];
   const VolTab &v = vt[blockIdx.x];
+  const Bufs b = bufs_at(vt, v.b);
   double acc = 0.0;
-  for (int i = threadIdx.x; i < v.nparts; i += blockDim.x) acc += v.b.parts[i];
+  for (int i = threadIdx.x; i < v.nparts; i += blockDim.x) acc += b.parts[i];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int k = blockDim.x / 2; k > 0; k >>= 1) {
@@ -2522,8 +2616,8 @@ __global__ void __launch_bounds__(256) kb_finish(const VolTab *__restrict__ vt) 
   }
   if (threadIdx.x == 0) {
     const float f = (float)sh[0];
-    v.b.scal[1] = f;
-    v.b.scal[2] = v.b.scal[0] / f;
+    b.scal[1] = f;
+    b.scal[2] = b.scal[0] / f;
   }
 }
 
@@ -2565,20 +2659,23 @@ class FastGroup : public RLGroup {
           L->fn = gr.first.fn;
           L->block = gr.first.block;
           std::vector<VolTab> tab;
-          std::vector<unsigned> map;
+          std::vector<int> blk0s;
+          unsigned grid = 0;
           for (int v : gr.second) {
             VolTab t = e_[v]->table_entry();
-            t.blk0 = (int)map.size();
-            const unsigned grid = e_[v]->batched(k).grid;
-            map.insert(map.end(), grid, (unsigned)tab.size());
+            t.blk0 = (int)grid;
+            blk0s.push_back(t.blk0);
+            grid += e_[v]->batched(k).grid;
             tab.push_back(t);
           }
-          L->grid = (unsigned)map.size();
+          L->grid = grid;
+          L->nvol = (int)tab.size();
           CL_TRY(L->tab.alloc(sizeof(VolTab) * tab.size()));
-          CL_TRY(L->map.alloc(sizeof(unsigned) * map.size()));
+          CL_TRY(L->blk0s.alloc(sizeof(int) * blk0s.size()));
+          for (VolTab &t : tab) t = relative_to(t, L->tab.p);
           CL_HIP(hipMemcpy(L->tab.p, tab.data(), sizeof(VolTab) * tab.size(),
                            hipMemcpyHostToDevice));
-          CL_HIP(hipMemcpy(L->map.p, map.data(), sizeof(unsigned) * map.size(),
+          CL_HIP(hipMemcpy(L->blk0s.p, blk0s.data(), sizeof(int) * blk0s.size(),
                            hipMemcpyHostToDevice));
           launches_[k][half].push_back(std::move(L));
         }
@@ -2586,6 +2683,7 @@ class FastGroup : public RLGroup {
     std::vector<VolTab> fin;
     for (FastRL *e : e_) fin.push_back(e->table_entry());
     CL_TRY(fin_.alloc(sizeof(VolTab) * fin.size()));
+    for (VolTab &t : fin) t = relative_to(t, fin_.p);
     CL_HIP(hipMemcpy(fin_.p, fin.data(), sizeof(VolTab) * fin.size(), hipMemcpyHostToDevice));
     return CLARITY_SUCCESS;
   }
@@ -2611,14 +2709,16 @@ class FastGroup : public RLGroup {
   struct Launch {
     const void *fn = nullptr;
     unsigned grid = 0, block = 0;
-    DevBuf tab, map;
+    int nvol = 0;
+    DevBuf tab, blk0s;  // volumes' entries; their first blocks, ascending
   };
 
   ClarityResult_t pass(Kind k, int half, hipStream_t s) {
     for (auto &L : launches_[k][half]) {
       const VolTab *t = static_cast<const VolTab *>(L->tab.p);
-      const unsigned *m = static_cast<const unsigned *>(L->map.p);
-      void *args[] = {&t, &m};
+      const int *b = static_cast<const int *>(L->blk0s.p);
+      int n = L->nvol;
+      void *args[] = {&t, &b, &n};
       CL_HIP(hipLaunchKernel(L->fn, dim3(L->grid), dim3(L->block), args, 0, s));
     }
     return CLARITY_SUCCESS;
