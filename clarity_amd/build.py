@@ -68,7 +68,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         if err and verbose:
             print(err, file=sys.stderr)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+    # -z defs: an unresolved symbol (e.g. a kernel whose host stub the
+    # compiler did not emit) fails the link instead of the first dlopen
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", tmp,
            *[o for o, _ in results]]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
