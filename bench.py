@@ -189,11 +189,17 @@ def suite(api, threads):
     obs = synth.poisson(api.Convolve(obj, psf), rng)
 
     def gpu_time(fn, reps=3):
+        # the CPU oracle's OpenMP/MKL threads keep spinning for a while after
+        # their last parallel region and would compete with the library's
+        # host-side copies: let them go idle first
+        time.sleep(0.5)
         fn()
-        t0 = time.perf_counter()
+        ts = []
         for _ in range(reps):
+            t0 = time.perf_counter()
             fn()
-        return (time.perf_counter() - t0) / reps
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]  # median
 
     def cpu_time(fn):
         t0 = time.perf_counter()
@@ -250,10 +256,10 @@ def suite(api, threads):
     # 48 volumes, x,y in {64..256}, z in {32..96}, own Gaussian kernels
     imgs, ks = synth.make_batch(convolve=api.Convolve)
     vox = sum(int(np.prod(a.shape)) for a in imgs)
-    g = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 50), reps=1)
+    g = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 50))
     # the same call with no iterations: plans, uploads, kernel transforms,
     # downloads; the difference is the 50 grouped iterations alone
-    g0 = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 0), reps=1)
+    g0 = gpu_time(lambda: api.MaximumLikelihoodDeconvolveBatch(imgs, ks, 0))
     t_cpu = 0.0
     for a, k in zip(imgs[:3], ks[:3]):
         oracle.ml(a, k, 1)
