@@ -637,6 +637,26 @@ template <> struct Plan<192, 12> { using Rad = Radices<12, 4, 4>; };
 template <> struct Plan<384, 12> { using Rad = Radices<6, 4, 4, 4>; };
 template <> struct Plan<768, 12> { using Rad = Radices<12, 4, 4, 4>; };
 
+// Radix plans of the x passes, whose rows sit in LDS with one pad slot per
+// 8 elements: a stage writes its outputs at positions base + r * P (span P
+// so far, r < R), which stay affine in r after padding -- one address
+// register plus immediate offsets -- only when P is a multiple of 8 or P *
+// R stays inside one pad period (P = 1 with R <= 4; P = 4 with R = 4).  A
+// radix 6 or 12 stage first breaks that for the first two stages (3-4
+// integer instructions per element written); taking it last instead costs
+// its R - 1 twiddle products per butterfly and leaves every write affine:
+// 384-point rows 1543 -> 1384 VALU instructions per wave in x_mid
+// (113-116 -> 109 us at 512x512x128), 192-point rows 1338 -> 1240.  (At 96
+// points, 8 lanes a row, it changes nothing.)  The y and z passes keep Plan (their
+// LDS is unpadded, so radix 6 or 12 first costs no addressing and saves
+// the twiddles).
+template <int M, int E>
+struct XPlan {
+  using Rad = typename Plan<M, E>::Rad;
+};
+template <> struct XPlan<192, 12> { using Rad = Radices<4, 4, 12>; };
+template <> struct XPlan<384, 12> { using Rad = Radices<4, 4, 4, 6>; };
+
 // Entries of an N-point twiddle table that fft_pruned reads for a radix
 // list: a stage of radix R at span P reads entry k * N / (P R) (R = 3, 4:
 // one read, powers by products) or r k N / (P R), r < R (other radices),
@@ -1088,7 +1108,7 @@ __device__ __forceinline__ void x_fwd_body(int blk, const float *__restrict__ u,
   tn.store(sm.twN);
   __syncthreads();
   const RowLds<PS> L{sm.tile + q * kRowStride<M, PS>};
-  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
@@ -1144,7 +1164,7 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
-  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
   // ratio r = guard(obs / s1) with s1 = c * (H (*) u) the blurred estimate;
   // |s1| < 1e-5 (the reference compares in double; kDivGuard is the same
   // test in float) gives 0, as the reference's ML step
@@ -1166,7 +1186,7 @@ __device__ __forceinline__ void x_mid_body(int blk, float2 *__restrict__ S, cons
     }
     v[e] = make_float2(r0, r1);
   }
-  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
@@ -1242,7 +1262,7 @@ __device__ __forceinline__ void x_last_body(int blk, const float2 *__restrict__ 
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
-  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
   if constexpr (MODE == 2) {
 #pragma unroll
     for (int e = 0; e < E; e++) {
@@ -1346,7 +1366,7 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
   float2 v[E];
   pack_inverse<M, E>(v, L, t, sm.twN);
   L.sync();  // the row's own lanes read it; the FFT overwrites it next
-  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, +1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
   // the same update (and for RL the same sum) as x_last
   const float c = cptr[0];
   double acc = 0.0;
@@ -1369,7 +1389,7 @@ __device__ __forceinline__ void x_last_fwd_body(int blk, float2 *__restrict__ S,
     }
     v[e] = make_float2(u0, u1);
   }
-  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename Plan<M, E>::Rad{});
+  fft_stages<M, E, 1, -1>(v, t, L, sm.twM, typename XPlan<M, E>::Rad{});
 #pragma unroll
   for (int e = 0; e < E; e++) L(t + T * e) = v[e];
   __syncthreads();
