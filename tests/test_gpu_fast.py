@@ -231,6 +231,28 @@ def test_fast_timing_api(clarity_gpu):
     plan.close()
 
 
+@pytest.mark.parametrize("shape", [(128, 512, 512), (32, 64, 64)])
+def test_x_tile_order_leaves_results_unchanged(clarity_gpu, shape):
+    """The x passes take their tiles from the last plane down by default
+    (xr=1) and from the first up with xr=0: each tile computes the same
+    values and writes its partial sum to its own slot, so the RL result is
+    bit for bit the same (512x512x128: 16-row tiles; 64x64x32: 8-row tiles
+    paired per 128-byte span)."""
+    api = clarity_gpu
+    rng = np.random.default_rng(9)
+    obs = rng.uniform(1.0, 100.0, shape).astype(np.float32)
+    psf = synth.gaussian_kernel(shape, 2.0, 4.5, side_lobe=0.1)
+    outs = []
+    for tune in ("xr=0", "xr=1"):
+        os.environ["CLARITY_FAST_TUNE"] = tune
+        try:
+            outs.append(api.MaximumLikelihoodDeconvolve(obs, psf, 3))
+            assert api.LastEngine() == "fast"
+        finally:
+            os.environ.pop("CLARITY_FAST_TUNE", None)
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("tune,exact", [("xh=0", True), (None, False)])
 def test_fused_x_boundary_matches_split(clarity_gpu, tune, exact):
     """Iterations chained through x_last_fwd equal the same iterations run
