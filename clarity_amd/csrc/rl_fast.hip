@@ -466,15 +466,6 @@ constexpr int kPadShift = (E == 16) ? 4 : 3;
 template <int M, int PS>
 constexpr int kRowStride = kPadded<M, PS> + 1;
 
-// Slot of position pos0 + off when the pad slots before it split into
-// pos0's and off's: (pos0 mod 2^PS) + (off mod 2^PS) < 2^PS, e.g. off a
-// multiple of the period.  The compiler cannot prove that split itself;
-// written this way the element's address is pos0's plus a constant.
-template <int PS>
-__device__ __forceinline__ int padded_at(int pos0, int off) {
-  return padded<PS>(pos0) + off + (off >> PS);
-}
-
 template <int PS>
 struct RowLds {
   float2 *base;
@@ -948,24 +939,15 @@ struct SpectrumRegs {
     const int qq = threadIdx.x % TP;
     const int k0 = threadIdx.x / TP;
     float2 *row = lds + qq * kRowStride<M, PS>;
-    // steps of whole pad periods: one address and constant offsets
-    constexpr bool SPLIT = KS % (1 << PS) == 0;
 #pragma unroll
     for (int it = 0; it < IT; it++) {
       const int k = k0 + it * KS;
-      // k == 0 only at it == 0; k >= M only at the last it; every k in
-      // between is a plain slot
-      if ((it == 0 && k == 0) || (it + 1 == IT && k == M)) {
-        if (k == 0)
-          row[0].x = X[it].x;
-        else
-          row[0].y = X[it].x;
-      } else if (it + 1 < IT || k < M) {
-        if constexpr (SPLIT)
-          row[padded_at<PS>(k0, it * KS)] = X[it];
-        else
-          row[padded<PS>(k)] = X[it];
-      }
+      if (k == 0)
+        row[0].x = X[it].x;
+      else if (k == M)
+        row[0].y = X[it].x;
+      else if (k < M)
+        row[padded<PS>(k)] = X[it];
     }
   }
 };
