@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -234,6 +235,281 @@ __global__ void __launch_bounds__(256)
   const int y = y0 + wave;
   if (y >= v.H) return;  // a whole wavefront (one row)
   conv_epilogue<COUT, HEAD>(acc, bias, Y, h, v, x0, y, z, m, kq);
+}
+
+// Staging of one input plane (a 66 x 6 patch of a 16-channel chunk) for the
+// plane-by-plane convolution below: two LDS buffers of [octet][voxel].
+constexpr int kPV = kTX * kTY;               // 396 voxels per staged plane
+constexpr int kPL = (2 * kPV + 255) / 256;   // 16-byte loads per thread and plane
+
+// The thread's staging slots of a plane: byte offsets within a plane (x, y
+// clamped into the volume; for an UP input also those into the coarse
+// tensor's plane) and which slots lie inside the volume in x and y.  They
+// are the same for every plane and chunk; a plane's base address is
+// uniform, so each load is one scalar base plus one 32-bit offset.
+struct ZmSlots {
+  unsigned off[kPL], offc[kPL];
+  unsigned inxy;  // bit j: slot j inside the volume in x and y
+};
+
+template <int CIN, int CA, bool UP>
+__device__ __forceinline__ ZmSlots zm_slots(Vol v, int x0, int y0) {
+  ZmSlots t;
+  t.inxy = 0;
+  constexpr int CF = UP ? CIN - CA : CIN;  // channels of the full-resolution tensor
+#pragma unroll
+  for (int j = 0; j < kPL; j++) {
+    const int i0 = threadIdx.x + 256 * j;
+    const int i = i0 < 2 * kPV ? i0 : 2 * kPV - 1;
+    const int vox = i >> 1, half = i & 1;
+    const int tx = vox % kTX, ty = vox / kTX;
+    const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+    const bool in = (gx >= 0) & (gx < v.W) & (gy >= 0) & (gy < v.H);
+    const int cx = min(max(gx, 0), v.W - 1), cy = min(max(gy, 0), v.H - 1);
+    t.inxy |= (in ? 1u : 0u) << j;
+    t.off[j] = 2u * ((unsigned)(cy * v.W + cx) * CF + 8 * half);
+    t.offc[j] = UP ? 2u * ((unsigned)((cy >> 1) * (v.W >> 1) + (cx >> 1)) * CA + 8 * half) : 0u;
+  }
+  return t;
+}
+
+// the thread's share of input plane z = gz of chunk c0, masked to zero
+// outside the volume
+template <int CIN, int CA, bool UP>
+__device__ __forceinline__ void zm_load(uint4 (&r)[kPL], const ZmSlots &t,
+                                        const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+                                        Vol v, int gz, int c0) {
+  const bool coarse = UP && c0 < CA;
+  const bool zin = (gz >= 0) & (gz < v.D);
+  const long long cz = min(max(gz, 0), v.D - 1);
+  const long long plane = (long long)v.H * v.W;
+  const char *pb;
+  if (!UP)
+    pb = reinterpret_cast<const char *>(X + cz * plane * CIN + c0);
+  else if (coarse)
+    pb = reinterpret_cast<const char *>(X + (cz >> 1) * (plane >> 2) * CA + c0);
+  else
+    pb = reinterpret_cast<const char *>(S + cz * plane * (CIN - CA) + (c0 - CA));
+#pragma unroll
+  for (int j = 0; j < kPL; j++) {
+    const uint4 val = *reinterpret_cast<const uint4 *>(pb + (coarse ? t.offc[j] : t.off[j]));
+    const unsigned m = (zin & ((t.inxy >> j) & 1u)) ? 0xffffffffu : 0u;
+    r[j] = make_uint4(val.x & m, val.y & m, val.z & m, val.w & m);
+  }
+}
+
+__device__ __forceinline__ void zm_store(const uint4 (&r)[kPL], uint4 *ring, int slot) {
+#pragma unroll
+  for (int j = 0; j < kPL; j++) {
+    const int i = threadIdx.x + 256 * j;
+    if (i < 2 * kPV) ring[(slot * 2 + (i & 1)) * kPV + (i >> 1)] = r[j];
+  }
+}
+
+// Per-lane constants of an output tile's epilogue (16 output channels):
+// the bias of the lane's four channels and, for the head, its weights, bias
+// and 1/mean -- loaded once, before the main loop, so that no load is
+// issued between the tile's stores (each would wait for the stores before it).
+struct Epi16 {
+  float4 bb, ww;
+  float hb, im;
+};
+
+template <bool HEAD>
+__device__ __forceinline__ Epi16 epi16_load(const float *__restrict__ bias, const HeadArgs &h,
+                                            int kq) {
+  Epi16 e;
+  e.bb = *reinterpret_cast<const float4 *>(bias + 4 * kq);
+  if constexpr (HEAD) {
+    e.ww = *reinterpret_cast<const float4 *>(h.w + 4 * kq);
+    e.hb = h.w[16];
+    e.im = h.inv_mean[0];
+  } else {
+    e.ww = float4{0.f, 0.f, 0.f, 0.f};
+    e.hb = e.im = 0.f;
+  }
+  return e;
+}
+
+// Epilogue of NP consecutive output planes (z0 ..) of one row: as
+// conv_epilogue with COUT = 16; the head's observation loads are issued
+// before any store.
+template <int NP, bool HEAD>
+__device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi16 &e,
+                                            bf16 *__restrict__ Y, const HeadArgs &h, Vol v,
+                                            int x0, int y, int z0, int m, int kq) {
+  if constexpr (HEAD) {
+    float ov[NP][4];
+#pragma unroll
+    for (int o = 0; o < NP; o++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const int xo = min(x0 + mt * 16 + m, v.W - 1), zo = min(z0 + o, v.D - 1);
+        ov[o][mt] = h.obs[((long long)zo * v.H + y) * v.W + xo];
+      }
+#pragma unroll
+    for (int o = 0; o < NP; o++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const float d0 = bf2f(f2bf(relu(acc[o][mt][0] + e.bb.x)));
+        const float d1 = bf2f(f2bf(relu(acc[o][mt][1] + e.bb.y)));
+        const float d2 = bf2f(f2bf(relu(acc[o][mt][2] + e.bb.z)));
+        const float d3 = bf2f(f2bf(relu(acc[o][mt][3] + e.bb.w)));
+        float p = e.ww.x * d0 + e.ww.y * d1 + e.ww.z * d2 + e.ww.w * d3;
+        p += __shfl_xor(p, 16);
+        p += __shfl_xor(p, 32);
+        const int xo = x0 + mt * 16 + m;
+        if (kq == 0 && xo < v.W && z0 + o < v.D) {
+          float r = ov[o][mt] * e.im + (e.hb + p);
+          if (h.rescale) r = (r > 0.f ? r : 0.f) / e.im;
+          h.out[((long long)(z0 + o) * v.H + y) * v.W + xo] = r;
+        }
+      }
+  } else {
+#pragma unroll
+    for (int o = 0; o < NP; o++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const int xo = x0 + mt * 16 + m;
+        if (xo < v.W && z0 + o < v.D) {
+          const float r0 = relu(acc[o][mt][0] + e.bb.x), r1 = relu(acc[o][mt][1] + e.bb.y);
+          const float r2 = relu(acc[o][mt][2] + e.bb.z), r3 = relu(acc[o][mt][3] + e.bb.w);
+          uint2 q;
+          q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
+          q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+          *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * 16 +
+                                     4 * kq) = q;
+        }
+      }
+  }
+}
+
+template <int N, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<N, I + 1>(f);
+  }
+}
+
+constexpr int kIB = 4;        // output planes per block
+constexpr int kIPDepth = 2;   // units whose loads are in flight
+constexpr bool kIPWLds = true;   // weights in LDS also for one chunk
+
+template <int CIN, int CA, bool UP, bool HEAD>
+__global__ void __launch_bounds__(256)
+    k_conv_ip(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+              const bf16 *__restrict__ Wf, const float *__restrict__ bias,
+              bf16 *__restrict__ Y, Vol v, HeadArgs h) {
+  static_assert(CIN % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)), "channel chunks of 16");
+  constexpr int COUT = 16, NCH = CIN / 16, NU = (kIB + 2) * NCH;
+  constexpr bool WLDS = NCH > 1 || kIPWLds;
+  __shared__ uint4 buf[2 * 2 * kPV];
+  __shared__ uint4 wl[WLDS ? NCH * 15 * 64 : 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int oct = kq & 1, e = kq >> 1;
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z0 = blockIdx.z * kIB;
+  // weight fragment of chunk c, offset dz, pair p for this lane
+  auto wfrag = [&](int c, int dz, int p) -> uint4 {
+    const int t = 2 * p + e;
+    const int tap = (dz + 1) * 9 + (t < 9 ? t : 8);
+    const uint4 wv = *reinterpret_cast<const uint4 *>(
+        Wf + ((long long)tap * COUT + m) * CIN + 16 * c + 8 * oct);
+    const unsigned z = t < 9 ? 0xffffffffu : 0u;
+    return make_uint4(wv.x & z, wv.y & z, wv.z & z, wv.w & z);
+  };
+  bf16x8 w[WLDS ? 1 : 15];
+  if constexpr (WLDS) {
+    for (int i = threadIdx.x; i < NCH * 15 * 64; i += 256) {
+      const int ln = i & 63, f = i >> 6;  // f = (c * 3 + dz + 1) * 5 + p
+      const int p = f % 5, dz = (f / 5) % 3 - 1, c = f / 15;
+      const int mm = ln & 15, kk = ln >> 4;
+      const int t = 2 * p + (kk >> 1);
+      const int tap = (dz + 1) * 9 + (t < 9 ? t : 8);
+      const uint4 wv = *reinterpret_cast<const uint4 *>(
+          Wf + ((long long)tap * COUT + mm) * CIN + 16 * c + 8 * (kk & 1));
+      const unsigned z = t < 9 ? 0xffffffffu : 0u;
+      wl[i] = make_uint4(wv.x & z, wv.y & z, wv.z & z, wv.w & z);
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 15; f++) w[f] = __builtin_bit_cast(bf16x8, wfrag(0, f / 5 - 1, f % 5));
+  }
+  // the lane's fragment offset within a buffer for pair p (in-plane tap of
+  // its lane half; the zero tenth tap reads tap 8's voxels)
+  int foff[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const int t0 = 2 * p, t1 = 2 * p + 1 < 9 ? 2 * p + 1 : 8;
+    const int o0 = (t0 / 3) * kTX + t0 % 3, o1 = (t1 / 3) * kTX + t1 % 3;
+    foff[p] = oct * kPV + wave * kTX + m + (e ? o1 : o0);
+  }
+  const ZmSlots t = zm_slots<CIN, CA, UP>(v, x0, y0);
+  const int y = y0 + wave;
+  const Epi16 ep = epi16_load<HEAD>(bias, h, kq);
+  f32x4 acc[kIB][4];
+  uint4 r[kIPDepth][kPL];
+#pragma unroll
+  for (int u = 0; u < kIPDepth && u < NU; u++)
+    zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 + u / NCH - 1, 16 * (u % NCH));
+  // units one by one at compile time (a loop of 18 is not always unrolled,
+  // and a unit index known only at run time would index register arrays)
+  static_for<NU>([&](auto uc) __attribute__((always_inline)) {
+    constexpr int u = decltype(uc)::value;
+    constexpr int s = u / NCH, c = u % NCH;  // input plane z0 - 1 + s, chunk c
+    zm_store(r[u % kIPDepth], buf, u & 1);
+    __syncthreads();  // the unit is in LDS; (WLDS) so are the weights
+    if constexpr (u + kIPDepth < NU)
+      zm_load<CIN, CA, UP>(r[u % kIPDepth], t, X, S, v, z0 + (u + kIPDepth) / NCH - 1,
+                           16 * ((u + kIPDepth) % NCH));
+    if constexpr (c == 0 && s < kIB) {
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) acc[s][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const uint4 *bb = buf + (u & 1) * 2 * kPV;
+    // software pipeline over the five pairs: pair p + 1's LDS reads are
+    // issued before pair p's MFMAs (scheduling barriers keep that order;
+    // left alone, the compiler reads each fragment just before its MFMA
+    // and waits out the LDS latency every time)
+    bf16x8 fa[2][4], wa[2][3];
+#define CLARITY_IP_READ_PAIR(P, B)                                                        \
+  {                                                                                       \
+    _Pragma("unroll") for (int mt = 0; mt < 4; mt++) fa[B][mt] =                          \
+        __builtin_bit_cast(bf16x8, bb[foff[P] + mt * 16]);                                \
+    _Pragma("unroll") for (int dz = -1; dz <= 1; dz++) {                                  \
+      const int o = s - 1 - dz; /* output plane z0 + o reads this plane at dz */          \
+      if (o < 0 || o >= kIB) continue;                                                    \
+      if constexpr (WLDS)                                                                 \
+        wa[B][dz + 1] = __builtin_bit_cast(bf16x8, wl[((c * 3 + dz + 1) * 5 + (P)) * 64 + lane]); \
+      else                                                                                \
+        wa[B][dz + 1] = w[(dz + 1) * 5 + (P)];                                            \
+    }                                                                                     \
+  }
+    CLARITY_IP_READ_PAIR(0, 0)
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      if (p + 1 < 5) CLARITY_IP_READ_PAIR(p + 1, (p + 1) & 1)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const int o = s - 1 - dz;
+        if (o < 0 || o >= kIB) continue;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+          acc[o][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[p & 1][dz + 1], fa[p & 1][mt],
+                                                               acc[o][mt], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#undef CLARITY_IP_READ_PAIR
+  });
+  // epilogues after the last unit: a store inside the unit loop, behind the
+  // row/plane bounds branch, would make the wait for a unit's loads also
+  // wait for the loads issued after it (vmcnt counts loads and stores in
+  // order, and the count past a branch is taken as the smaller one)
+  if (y >= v.H) return;
+  epi16_store<kIB, HEAD>(acc, ep, Y, h, v, x0, y, z0, m, kq);
 }
 
 // The first convolution for inference: the network input (obs/mean,
@@ -489,22 +765,42 @@ __global__ void k_reduce_chunks(const float *__restrict__ P, int nchunks, long l
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
        i += (long long)gridDim.x * blockDim.x)
 
-// 2x2x2 average pooling; v is the fine volume
-__global__ void k_pool(const bf16 *__restrict__ X, bf16 *__restrict__ Y, Vol v, int C) {
-  const int D2 = v.D / 2, H2 = v.H / 2, W2 = v.W / 2;
-  const long long n = (long long)D2 * H2 * W2 * C;
+// 2x2x2 average pooling; v is the fine volume.  One thread per output voxel
+// and octet of channels: eight 16-byte loads, one 16-byte store; each
+// channel sums its eight inputs in the fixed order k = 0..7 (z, y, x bits).
+// Pooled voxel indices fit in 32 bits (2^32 of them would be a 1 TB input).
+template <int C>
+__global__ void __launch_bounds__(256) k_pool(const bf16 *__restrict__ X, bf16 *__restrict__ Y,
+                                              Vol v) {
+  static_assert(C % 8 == 0, "channel octets");
+  constexpr int NO = C / 8;
+  const unsigned D2 = v.D / 2, H2 = v.H / 2, W2 = v.W / 2;
+  const long long n = (long long)D2 * H2 * W2 * NO;
   VOX_LOOP(i, n) {
-    const int c = (int)(i % C);
-    long long p = i / C;
-    const int x = (int)(p % W2);
-    p /= W2;
-    const int y = (int)(p % H2), z = (int)(p / H2);
-    float s = 0.f;
+    const int oc = (int)(i % NO);
+    const unsigned p = (unsigned)(i / NO);
+    const unsigned x = p % W2, q = p / W2;
+    const unsigned y = q % H2, z = q / H2;
+    uint4 in[8];
+#pragma unroll
     for (int k = 0; k < 8; k++) {
-      const int zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
-      s += bf2f(X[(((long long)zz * v.H + yy) * v.W + xx) * C + c]);
+      const long long zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
+      in[k] = *reinterpret_cast<const uint4 *>(X + ((zz * v.H + yy) * v.W + xx) * C + 8 * oc);
     }
-    Y[i] = f2bf(0.125f * s);
+    float sum[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) sum[e] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const unsigned w4[4] = {in[k].x, in[k].y, in[k].z, in[k].w};
+#pragma unroll
+      for (int e = 0; e < 8; e++) sum[e] += bf2f((bf16)(w4[e >> 1] >> (16 * (e & 1))));
+    }
+    unsigned o4[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      o4[j] = (unsigned)f2bf(0.125f * sum[2 * j]) | ((unsigned)f2bf(0.125f * sum[2 * j + 1]) << 16);
+    *reinterpret_cast<uint4 *>(Y + (long long)p * C + 8 * oc) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
   }
 }
 
@@ -691,16 +987,21 @@ constexpr int kLevel[10] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
 template <int CIN, int COUT, int CA = 0, bool UP = false>
 void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
                 hipStream_t s) {
-  hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
-                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
-                     X, S, Wf, b, Y, v, HeadArgs{});
+  if constexpr (COUT == 16)
+    hipLaunchKernelGGL((k_conv_ip<CIN, CA, UP, false>),
+                       dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + kIB - 1) / kIB),
+                       dim3(256), 0, s, X, S, Wf, b, Y, v, HeadArgs{});
+  else
+    hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
+                       dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
+                       X, S, Wf, b, Y, v, HeadArgs{});
 }
 // the last convolution (16 -> 16) with the head fused into its epilogue
 void conv_fwd_head(const bf16 *X, const bf16 *Wf, const float *b, const HeadArgs &h, Vol v,
                    hipStream_t s) {
-  hipLaunchKernelGGL((k_conv_fwd<16, 16, 0, false, true>),
-                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
-                     X, nullptr, Wf, b, nullptr, v, h);
+  hipLaunchKernelGGL((k_conv_ip<16, 0, false, true>),
+                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + kIB - 1) / kIB),
+                     dim3(256), 0, s, X, nullptr, Wf, b, nullptr, v, h);
 }
 template <int CIN, int COUT>
 void conv_bwd_t(const bf16 *G, const bf16 *Yo, const bf16 *X, const bf16 *Wb, bf16 *GX,
@@ -804,6 +1105,9 @@ class UNet {
   ClarityResult_t shape(Vol v, bool train) {
     if (v.D % 4 || v.H % 4 || v.W % 4 || v.D < 4 || v.H < 4 || v.W < 4)
       return CLARITY_INVALID_ARGUMENT;
+    // k_conv_ip addresses a plane of up to 48 bfloat16 channels with 32-bit
+    // byte offsets
+    if ((long long)v.H * v.W * 48 * 2 >= (1LL << 32)) return CLARITY_INVALID_ARGUMENT;
     const bool same = v.D == v0_.D && v.H == v0_.H && v.W == v0_.W;
     if (same && (!train || train_ready_)) return CLARITY_SUCCESS;
     const long long n[3] = {v.n(), v.n() / 8, v.n() / 64};
@@ -885,11 +1189,11 @@ class UNet {
     src[9] = (bf16 *)act_[8].p;
     for (int i = 0; i < 10; i++) {
       if (i == 2)
-        hipLaunchKernelGGL(k_pool, dim3(grid_for(v1.n() * 16)), dim3(256), 0, s,
-                           (bf16 *)act_[1].p, (bf16 *)pool1_.p, v, 16);
+        hipLaunchKernelGGL(k_pool<16>, dim3(grid_for(v1.n() * 2)), dim3(256), 0, s,
+                           (bf16 *)act_[1].p, (bf16 *)pool1_.p, v);
       if (i == 4)
-        hipLaunchKernelGGL(k_pool, dim3(grid_for(v2.n() * 32)), dim3(256), 0, s,
-                           (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1, 32);
+        hipLaunchKernelGGL(k_pool<32>, dim3(grid_for(v2.n() * 4)), dim3(256), 0, s,
+                           (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1);
       bf16 *Y = (bf16 *)act_[i].p;
       const float *b = p_.f() + off_[10 + i];
       const bf16 *Wf = (bf16 *)wf_[i].p;
