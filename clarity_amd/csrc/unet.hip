@@ -512,6 +512,127 @@ __global__ void __launch_bounds__(256)
   epi16_store<kIB, HEAD>(acc, ep, Y, h, v, x0, y, z0, m, kq);
 }
 
+constexpr int kCoDepth = 1;  // units whose loads are in flight (k_conv_co)
+
+// The same plane-by-plane scheme for more output channels (COUT/16 tiles,
+// each fragment feeding up to 3 x COUT/16 MFMAs), chunk-outer: the chunk's
+// weights (COUT/16 x 15 KB) sit in LDS while its IB + 2 input planes pass,
+// and every output plane of the block keeps its accumulators to the end.
+// Per output voxel the sum runs over chunks, then input planes z-1, z, z+1,
+// then pairs.
+template <int CIN, int COUT, int CA, bool UP, int IB>
+__global__ void __launch_bounds__(256)
+    k_conv_co(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
+              const bf16 *__restrict__ Wf, const float *__restrict__ bias,
+              bf16 *__restrict__ Y, Vol v) {
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
+                "channel chunks of 16");
+  constexpr int NCT = COUT / 16, NCH = CIN / 16, NS = IB + 2, NU = NCH * NS;
+  constexpr int NW = NCT * 15 * 64;  // weight fragments of a chunk
+  __shared__ uint4 buf[2 * 2 * kPV];
+  __shared__ uint4 wl[NW];  // [dz][pair][ct][lane]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int oct = kq & 1, e = kq >> 1;
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z0 = blockIdx.z * IB;
+  int foff[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const int t0 = 2 * p, t1 = 2 * p + 1 < 9 ? 2 * p + 1 : 8;
+    const int o0 = (t0 / 3) * kTX + t0 % 3, o1 = (t1 / 3) * kTX + t1 % 3;
+    foff[p] = oct * kPV + wave * kTX + m + (e ? o1 : o0);
+  }
+  const ZmSlots t = zm_slots<CIN, CA, UP>(v, x0, y0);
+  const int y = y0 + wave;
+  float4 bb[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ct++) bb[ct] = *reinterpret_cast<const float4 *>(bias + ct * 16 + 4 * kq);
+  f32x4 acc[IB][NCT][4];
+#pragma unroll
+  for (int o = 0; o < IB; o++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) acc[o][ct][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 r[kCoDepth][kPL];
+#pragma unroll
+  for (int u = 0; u < kCoDepth && u < NU; u++)
+    zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 - 1 + u % NS, 16 * (u / NS));
+  static_for<NU>([&](auto uc) __attribute__((always_inline)) {
+    constexpr int u = decltype(uc)::value;
+    constexpr int c = u / NS, s = u % NS;  // chunk c, input plane z0 - 1 + s
+    zm_store(r[u % kCoDepth], buf, u & 1);
+    __syncthreads();  // the unit is in LDS; the previous chunk's MFMAs are done
+    if constexpr (s == 0) {
+      for (int i = threadIdx.x; i < NW; i += 256) {
+        const int ln = i & 63, f = i >> 6;  // f = ((dz + 1) * 5 + p) * NCT + ct
+        const int ct = f % NCT, p = (f / NCT) % 5, dz = f / (NCT * 5) - 1;
+        const int mm = ln & 15, kk = ln >> 4;
+        const int tt = 2 * p + (kk >> 1);
+        const int tap = (dz + 1) * 9 + (tt < 9 ? tt : 8);
+        const uint4 wv = *reinterpret_cast<const uint4 *>(
+            Wf + ((long long)tap * COUT + ct * 16 + mm) * CIN + 16 * c + 8 * (kk & 1));
+        const unsigned z = tt < 9 ? 0xffffffffu : 0u;
+        wl[i] = make_uint4(wv.x & z, wv.y & z, wv.z & z, wv.w & z);
+      }
+      __syncthreads();
+    }
+    if constexpr (u + kCoDepth < NU)
+      zm_load<CIN, CA, UP>(r[u % kCoDepth], t, X, S, v, z0 - 1 + (u + kCoDepth) % NS,
+                           16 * ((u + kCoDepth) / NS));
+    const uint4 *bq = buf + (u & 1) * 2 * kPV;
+    bf16x8 fa[2][4], wa[2][3][NCT];
+#define CLARITY_CO_READ_PAIR(P, B)                                                        \
+  {                                                                                       \
+    _Pragma("unroll") for (int mt = 0; mt < 4; mt++) fa[B][mt] =                          \
+        __builtin_bit_cast(bf16x8, bq[foff[P] + mt * 16]);                                \
+    _Pragma("unroll") for (int dz = -1; dz <= 1; dz++) {                                  \
+      const int o = s - 1 - dz;                                                           \
+      if (o < 0 || o >= IB) continue;                                                     \
+      _Pragma("unroll") for (int ct = 0; ct < NCT; ct++) wa[B][dz + 1][ct] =              \
+          __builtin_bit_cast(bf16x8, wl[(((dz + 1) * 5 + (P)) * NCT + ct) * 64 + lane]);  \
+    }                                                                                     \
+  }
+    CLARITY_CO_READ_PAIR(0, 0)
+#pragma unroll
+    for (int p = 0; p < 5; p++) {
+      if (p + 1 < 5) CLARITY_CO_READ_PAIR(p + 1, (p + 1) & 1)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const int o = s - 1 - dz;  // output plane z0 + o reads this plane at dz
+        if (o < 0 || o >= IB) continue;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+          for (int mt = 0; mt < 4; mt++)
+            acc[o][ct][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                wa[p & 1][dz + 1][ct], fa[p & 1][mt], acc[o][ct][mt], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#undef CLARITY_CO_READ_PAIR
+  });
+  if (y >= v.H) return;
+#pragma unroll
+  for (int o = 0; o < IB; o++)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        const int xo = x0 + mt * 16 + m;
+        if (xo < v.W && z0 + o < v.D) {
+          const float r0 = relu(acc[o][ct][mt][0] + bb[ct].x), r1 = relu(acc[o][ct][mt][1] + bb[ct].y);
+          const float r2 = relu(acc[o][ct][mt][2] + bb[ct].z), r3 = relu(acc[o][ct][mt][3] + bb[ct].w);
+          uint2 q;
+          q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
+          q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+          *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * COUT +
+                                     ct * 16 + 4 * kq) = q;
+        }
+      }
+}
+
 // The first convolution for inference: the network input (obs/mean,
 // sigma_xy/4, sigma_z/8, then 13 zero channels) is built in the staging
 // from the observation itself, and only its 4 leading channels are kept
@@ -984,6 +1105,7 @@ constexpr ConvSpec kConv[10] = {{16, 16}, {16, 16}, {16, 32}, {32, 32}, {32, 64}
                                 {64, 64}, {96, 32}, {32, 32}, {48, 16}, {16, 16}};
 constexpr int kLevel[10] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
 
+constexpr int kCoIB = 2;  // output planes per block of k_conv_co (32 channels)
 template <int CIN, int COUT, int CA = 0, bool UP = false>
 void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
                 hipStream_t s) {
@@ -991,6 +1113,10 @@ void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf
     hipLaunchKernelGGL((k_conv_ip<CIN, CA, UP, false>),
                        dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + kIB - 1) / kIB),
                        dim3(256), 0, s, X, S, Wf, b, Y, v, HeadArgs{});
+  else if constexpr (COUT == 32)
+    hipLaunchKernelGGL((k_conv_co<CIN, COUT, CA, UP, kCoIB>),
+                       dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + kCoIB - 1) / kCoIB),
+                       dim3(256), 0, s, X, S, Wf, b, Y, v);
   else
     hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
                        dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
