@@ -640,29 +640,46 @@ __global__ void __launch_bounds__(256)
 // g supplies taps 8p + 2g and 8p + 2g + 1, so 4 MFMAs cover the 27 taps
 // (5 with zero weights) where the 16-channel layout needs 14.  The same
 // products as k_conv_fwd on the materialised input (its other channels are
-// zero).  Weights Wf as k_conv_fwd's with CIN = 16.
-template <int COUT>
+// zero).  Weights Wf as k_conv_fwd's with CIN = 16 (16 output channels).
+// A block computes the 64 x 4 tile of IB consecutive z planes from its
+// IB + 2 input planes, all staged at once (IB = 8: 32 KB), so each input
+// voxel is converted (IB + 2) / IB times instead of three times.
+template <int IB>
 __global__ void __launch_bounds__(256)
     k_conv_in(const float *__restrict__ obs, const float *__restrict__ inv_mean, float sxy,
               float sz, const bf16 *__restrict__ Wf, const float *__restrict__ bias,
               bf16 *__restrict__ Y, Vol v) {
-  __shared__ uint2 tile[kTV];
+  constexpr int COUT = 16;
+  constexpr int NV = (IB + 2) * kPV;
+  __shared__ uint2 tile[NV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z = blockIdx.z;
+  const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z0 = blockIdx.z * IB;
   const float im = inv_mean[0];
   const unsigned w0hi = (unsigned)f2bf(sxy * 0.25f) << 16;  // channel 1
   const unsigned w1 = (unsigned)f2bf(sz * 0.125f);          // channel 2 (3: zero)
-  constexpr int NI = (kTV + 255) / 256;
+  const float4 bb = *reinterpret_cast<const float4 *>(bias + 4 * kq);
+  // weights of the four K = 32 groups (taps 8p + 2g, 8p + 2g + 1)
+  bf16x8 w[4];
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    const int ta = 8 * p + 2 * kq, tb = ta + 1;
+    const unsigned ma = ta < 27 ? 0xffffffffu : 0u, mb = tb < 27 ? 0xffffffffu : 0u;
+    const int ca = ta < 27 ? ta : 26, cb = tb < 27 ? tb : 26;
+    const uint2 wa = *reinterpret_cast<const uint2 *>(Wf + ((long long)ca * COUT + m) * 16);
+    const uint2 wb = *reinterpret_cast<const uint2 *>(Wf + ((long long)cb * COUT + m) * 16);
+    w[p] = __builtin_bit_cast(bf16x8, make_uint4(wa.x & ma, wa.y & ma, wb.x & mb, wb.y & mb));
+  }
+  constexpr int NI = (NV + 255) / 256;
   float ov[NI];
   bool inb[NI];
 #pragma unroll
   for (int j = 0; j < NI; j++) {  // all loads first (clamped, no branches)
     const int i0 = threadIdx.x + 256 * j;
-    const int i = i0 < kTV ? i0 : kTV - 1;
-    const int tx = i % kTX, t2 = i / kTX;
-    const int ty = t2 % kTY, tz = t2 / kTY;
-    const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z + tz - 1;
+    const int i = i0 < NV ? i0 : NV - 1;
+    const int tz = i / kPV, r = i - tz * kPV;
+    const int ty = r / kTX, tx = r - ty * kTX;
+    const int gx = x0 + tx - 1, gy = y0 + ty - 1, gz = z0 + tz - 1;
     inb[j] = (gx >= 0) & (gx < v.W) & (gy >= 0) & (gy < v.H) & (gz >= 0) & (gz < v.D);
     const int cx = min(max(gx, 0), v.W - 1), cy = min(max(gy, 0), v.H - 1);
     const int cz = min(max(gz, 0), v.D - 1);
@@ -672,43 +689,47 @@ __global__ void __launch_bounds__(256)
   for (int j = 0; j < NI; j++) {
     const int i = threadIdx.x + 256 * j;
     const unsigned mk = inb[j] ? 0xffffffffu : 0u;
-    if (i < kTV) tile[i] = make_uint2(((unsigned)f2bf(ov[j] * im) | w0hi) & mk, w1 & mk);
+    if (i < NV) tile[i] = make_uint2(((unsigned)f2bf(ov[j] * im) | w0hi) & mk, w1 & mk);
   }
   __syncthreads();
-  f32x4 acc[4][COUT / 16];
+  const int y = y0 + wave;
+  if (y >= v.H) return;  // a whole wavefront (one row); no barrier follows
 #pragma unroll
-  for (int mt = 0; mt < 4; mt++)
+  for (int o = 0; o < IB; o++) {
+    f32x4 acc[4];
 #pragma unroll
-    for (int ct = 0; ct < COUT / 16; ct++) acc[mt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int mt = 0; mt < 4; mt++) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int p = 0; p < 4; p++) {
-    const int ta = 8 * p + 2 * kq, tb = ta + 1;
-    const unsigned ma = ta < 27 ? 0xffffffffu : 0u, mb = tb < 27 ? 0xffffffffu : 0u;
-    const int ca = ta < 27 ? ta : 26, cb = tb < 27 ? tb : 26;
-    int dz, dy, dx;
-    tap_offset(ca, dz, dy, dx);
-    const int ra = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
-    tap_offset(cb, dz, dy, dx);
-    const int rb = ((dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
-    bf16x8 w[COUT / 16];
+    for (int p = 0; p < 4; p++) {
+      const int ta = 8 * p + 2 * kq, tb = ta + 1;
+      const int ca = ta < 27 ? ta : 26, cb = tb < 27 ? tb : 26;
+      int dz, dy, dx;
+      tap_offset(ca, dz, dy, dx);
+      const int ra = ((o + dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
+      tap_offset(cb, dz, dy, dx);
+      const int rb = ((o + dz + 1) * kTY + wave + dy + 1) * kTX + dx + 1;
 #pragma unroll
-    for (int ct = 0; ct < COUT / 16; ct++) {
-      const uint2 wa = *reinterpret_cast<const uint2 *>(Wf + ((long long)ca * COUT + ct * 16 + m) * 16);
-      const uint2 wb = *reinterpret_cast<const uint2 *>(Wf + ((long long)cb * COUT + ct * 16 + m) * 16);
-      w[ct] = __builtin_bit_cast(bf16x8, make_uint4(wa.x & ma, wa.y & ma, wb.x & mb, wb.y & mb));
+      for (int mt = 0; mt < 4; mt++) {
+        const uint2 a0 = tile[ra + mt * 16 + m], a1 = tile[rb + mt * 16 + m];
+        const bf16x8 a = __builtin_bit_cast(bf16x8, make_uint4(a0.x, a0.y, a1.x, a1.y));
+        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[p], a, acc[mt], 0, 0, 0);
+      }
     }
+    if (z0 + o >= v.D) break;
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      const uint2 a0 = tile[ra + mt * 16 + m], a1 = tile[rb + mt * 16 + m];
-      const bf16x8 a = __builtin_bit_cast(bf16x8, make_uint4(a0.x, a0.y, a1.x, a1.y));
-#pragma unroll
-      for (int ct = 0; ct < COUT / 16; ct++)
-        acc[mt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[ct], a, acc[mt][ct], 0, 0, 0);
+      const int xo = x0 + mt * 16 + m;
+      if (xo < v.W) {
+        const float r0 = relu(acc[mt][0] + bb.x), r1 = relu(acc[mt][1] + bb.y);
+        const float r2 = relu(acc[mt][2] + bb.z), r3 = relu(acc[mt][3] + bb.w);
+        uint2 q;
+        q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
+        q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+        *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * COUT +
+                                   4 * kq) = q;
+      }
     }
   }
-  const int y = y0 + wave;
-  if (y >= v.H) return;
-  conv_epilogue<COUT, false>(acc, bias, Y, HeadArgs{}, v, x0, y, z, m, kq);
 }
 
 // GX = conv^T(G * [Y > 0], W): input gradient of a ReLU convolution.
@@ -1106,6 +1127,7 @@ constexpr ConvSpec kConv[10] = {{16, 16}, {16, 16}, {16, 32}, {32, 32}, {32, 64}
 constexpr int kLevel[10] = {0, 0, 1, 1, 2, 2, 1, 1, 0, 0};
 
 constexpr int kCoIB = 2;  // output planes per block of k_conv_co (32 channels)
+constexpr int kInIB = 8;  // output planes per block of k_conv_in
 template <int CIN, int COUT, int CA = 0, bool UP = false>
 void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf16 *Y, Vol v,
                 hipStream_t s) {
@@ -1325,9 +1347,10 @@ class UNet {
       const bf16 *Wf = (bf16 *)wf_[i].p;
       const Vol vv = lv[kLevel[i]];
       if (i == 0 && head_out) {
-        hipLaunchKernelGGL((k_conv_in<16>),
-                           dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256),
-                           0, s, obs_d, invm_.f() + 1, sxy, sz, Wf, b, Y, v);
+        hipLaunchKernelGGL((k_conv_in<kInIB>),
+                           dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT,
+                                (v.D + kInIB - 1) / kInIB),
+                           dim3(256), 0, s, obs_d, invm_.f() + 1, sxy, sz, Wf, b, Y, v);
         continue;
       }
       if (i == 9 && head_out) {
