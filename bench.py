@@ -30,6 +30,8 @@ Also reported:
                 with plain RL's time to reach the same error.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+    python bench.py --suite      # every other driver on the path, one line each
+    python bench.py --network    # the learned warm start's network alone
 """
 from __future__ import annotations
 
@@ -170,6 +172,37 @@ def dense_bytes(shape, passes_real, passes_complex):
     return 4.0 * nx * ny * nz * passes_real + 8.0 * (nx // 2 + 1) * ny * nz * passes_complex
 
 
+def network_only(api, reps=7):
+    """The learned warm start's network alone: U-Net inference on the
+    512x512x128 filament volume (sigma_xy 2.0, sigma_z 4.5), observation
+    and estimate in device memory; median of `reps` after two warm-ups.
+    One JSON line (the per-kernel profile of the network runs this)."""
+    from clarity_amd import learned as L
+    shape = (128, 512, 512)
+    obs, _ = make_inputs(api, shape)
+    hip = Hip()
+    d_obs, d_out = hip.malloc(obs.nbytes), hip.malloc(obs.nbytes)
+    hip.h2d(d_obs, obs)
+    params, _ = L.load()
+    net = api.UNet()
+    net.set_params(params)
+    ts = []
+    for i in range(reps + 2):
+        api.Synchronize()
+        t = time.perf_counter()
+        net.predict_device(d_obs, shape, 2.0, 4.5, d_out)
+        api.Synchronize()
+        if i >= 2:
+            ts.append(time.perf_counter() - t)
+    net.close()
+    hip.free(d_obs)
+    hip.free(d_out)
+    print(json.dumps({"network": "learned warm start U-Net inference",
+                      "workload": "filaments 512x512x128", "reps": reps,
+                      "ms_median": round(1e3 * float(np.median(ts)), 3),
+                      "ms_min": round(1e3 * float(np.min(ts)), 3)}), flush=True)
+
+
 def suite(api, threads):
     """Every other driver on the path, GPU vs the CPU oracle on the same
     inputs (a 256x256x64 volume with a same-size kernel; batched RL on 48
@@ -303,7 +336,17 @@ def main():
     ap.add_argument("--suite", action="store_true",
                     help="instead of the headline line, time every other driver on "
                          "the path (GPU vs CPU oracle), one JSON line each")
+    ap.add_argument("--network", action="store_true",
+                    help="instead of the headline line, time the learned warm start's "
+                         "network alone (U-Net inference on the 512x512x128 volume)")
     args = ap.parse_args()
+
+    if args.network:
+        from clarity_amd import api
+        api.Register()
+        network_only(api)
+        api.UnRegister()
+        return
 
     if args.suite:
         from clarity_amd import api
