@@ -57,8 +57,10 @@ __device__ __forceinline__ void tap_offset(int tap, int &dz, int &dy, int &dx) {
 
 // ------------------------------------------------------------------ conv
 
-// Y = relu(conv(X, W) + b).  A block computes a 64 (x) by 4 (y) tile of
-// output voxels of one z plane, one wavefront per row, all COUT channels.
+// Y = relu(conv(X, W) + b), one plane per block (the 64-output-channel
+// layers; k_conv_ip and k_conv_co below take the others).  A block computes
+// a 64 (x) by 4 (y) tile of output voxels of one z plane, one wavefront per
+// row, all COUT channels.
 // Input channels go in chunks of 16: the chunk's (3 x 6 x 66)-voxel input
 // tile is staged in LDS as two planes of channel octets ([octet][voxel],
 // 16 B each, one 16-byte read per MFMA operand), and every tap reads its
@@ -87,39 +89,13 @@ struct HeadArgs {
 
 // Epilogue of an output tile: y = relu(acc + bias) for the wavefront's row
 // (D row = channel 4*(lane/16) + j, column = voxel lane%16 of each of the
-// four 16-voxel groups), stored as bfloat16 -- or, with HEAD (COUT = 16),
-// reduced over the channels by the head: each lane sums its 4 channels,
-// two lane exchanges sum the 4 lanes of a voxel.
-template <int COUT, bool HEAD>
+// four 16-voxel groups), stored as bfloat16.
+template <int COUT>
 __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[4][COUT / 16],
                                               const float *__restrict__ bias,
-                                              bf16 *__restrict__ Y, const HeadArgs &h, Vol v,
-                                              int x0, int y, int z, int m, int kq) {
+                                              bf16 *__restrict__ Y, Vol v, int x0, int y, int z,
+                                              int m, int kq) {
   const long long rowbase = ((long long)z * v.H + y) * v.W;
-  if constexpr (HEAD) {
-    static_assert(COUT == 16, "the head reads 16 channels");
-    const float im = h.inv_mean[0];
-    const float hb = h.w[16];
-    const float4 bb = *reinterpret_cast<const float4 *>(bias + 4 * kq);
-    const float4 ww = *reinterpret_cast<const float4 *>(h.w + 4 * kq);
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      const float d0 = bf2f(f2bf(relu(acc[mt][0][0] + bb.x)));
-      const float d1 = bf2f(f2bf(relu(acc[mt][0][1] + bb.y)));
-      const float d2 = bf2f(f2bf(relu(acc[mt][0][2] + bb.z)));
-      const float d3 = bf2f(f2bf(relu(acc[mt][0][3] + bb.w)));
-      float p = ww.x * d0 + ww.y * d1 + ww.z * d2 + ww.w * d3;
-      p += __shfl_xor(p, 16);
-      p += __shfl_xor(p, 32);
-      const int xo = x0 + mt * 16 + m;
-      if (kq == 0 && xo < v.W) {
-        float r = h.obs[rowbase + xo] * im + (hb + p);
-        if (h.rescale) r = (r > 0.f ? r : 0.f) / im;
-        h.out[rowbase + xo] = r;
-      }
-    }
-    return;
-  }
 #pragma unroll
   for (int ct = 0; ct < COUT / 16; ct++) {
     const int co = ct * 16 + 4 * kq;
@@ -139,11 +115,11 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[4][COUT / 16],
   }
 }
 
-template <int CIN, int COUT, int CA, bool UP, bool HEAD = false>
+template <int CIN, int COUT, int CA, bool UP>
 __global__ void __launch_bounds__(256)
     k_conv_fwd(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
                const bf16 *__restrict__ Wf, const float *__restrict__ bias,
-               bf16 *__restrict__ Y, Vol v, HeadArgs h) {
+               bf16 *__restrict__ Y, Vol v) {
   static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
                 "channel chunks of 16");
   __shared__ uint4 tile[2][kTV];
@@ -234,7 +210,7 @@ __global__ void __launch_bounds__(256)
   }
   const int y = y0 + wave;
   if (y >= v.H) return;  // a whole wavefront (one row)
-  conv_epilogue<COUT, HEAD>(acc, bias, Y, h, v, x0, y, z, m, kq);
+  conv_epilogue<COUT>(acc, bias, Y, v, x0, y, z, m, kq);
 }
 
 // Staging of one input plane (a 66 x 6 patch of a 16-channel chunk) for the
@@ -331,9 +307,11 @@ __device__ __forceinline__ Epi16 epi16_load(const float *__restrict__ bias, cons
   return e;
 }
 
-// Epilogue of NP consecutive output planes (z0 ..) of one row: as
-// conv_epilogue with COUT = 16; the head's observation loads are issued
-// before any store.
+// Epilogue of NP consecutive output planes (z0 ..) of one row: y =
+// relu(acc + bias) stored as bfloat16 (16 channels), or with HEAD reduced
+// over the channels by the head (each lane sums its 4 channels, two lane
+// exchanges sum the 4 lanes of a voxel); the head's observation loads are
+// issued before any store.
 template <int NP, bool HEAD>
 __device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi16 &e,
                                             bf16 *__restrict__ Y, const HeadArgs &h, Vol v,
@@ -1142,7 +1120,7 @@ void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf
   else
     hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
                        dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
-                       X, S, Wf, b, Y, v, HeadArgs{});
+                       X, S, Wf, b, Y, v);
 }
 // the last convolution (16 -> 16) with the head fused into its epilogue
 void conv_fwd_head(const bf16 *X, const bf16 *Wf, const float *b, const HeadArgs &h, Vol v,
