@@ -370,9 +370,27 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
+// Input-plane-major convolution for 16 output channels (the full-resolution
+// layers): a block computes the 64 x 4 tile of kIB consecutive z planes
+// and walks the kIB + 2 input planes it reads once each, chunk by chunk.
+// An input plane feeds three output planes (dz = -1, 0, 1) through the same
+// in-plane tap positions, so the K = 32 pairs are formed within one dz:
+// pair p, lane half e = kq / 2 holds in-plane tap t = 2p + e (dy = t / 3 -
+// 1, dx = t % 3 - 1; the tenth tap is zero), and each 16-voxel fragment
+// read from LDS feeds the three planes' MFMAs (60 MFMAs per 20 fragment
+// reads).  (plane, chunk) units go through two LDS buffers, the loads of
+// the unit kIPDepth ahead in flight during the current unit's MFMAs; pair
+// p + 1's LDS reads are issued before pair p's MFMAs.  Weights: one chunk
+// in registers (60 VGPRs), more chunks in LDS ([chunk][dz][pair][lane], 15
+// KB each).  Every output plane keeps its accumulators to the end and the
+// epilogues follow the unit loop (a store inside it, behind the bounds
+// branch, would make the wait for a unit's loads also wait for the loads
+// issued after it: vmcnt counts loads and stores in order).  Per output
+// voxel the sum runs over input planes z-1, z, z+1, then chunks, then
+// pairs.
 constexpr int kIB = 4;        // output planes per block
 constexpr int kIPDepth = 2;   // units whose loads are in flight
-constexpr bool kIPWLds = true;   // weights in LDS also for one chunk
+constexpr bool kIPWLds = false;  // weights in LDS also for one chunk
 
 template <int CIN, int CA, bool UP, bool HEAD>
 __global__ void __launch_bounds__(256)
