@@ -2300,17 +2300,27 @@ class FastRL : public RLEngine {
     CL_TRY(run(KX_FWD, s));
     for (int k = 0; k < 2; k++)
       if (!graph_exec_[k]) CL_TRY(capture(k, &graph_exec_[k], s));
-    for (int it = 0; it + 1 < n; it++) CL_HIP(hipGraphLaunch(graph_exec_[0], s));
+    // runs of kGraphRun chained bodies in one graph: successive graph
+    // launches leave the GPU idle ~9 us between them, the kernels inside a
+    // graph follow each other without a gap
+    int rest = n - 1;
+    if (rest >= kGraphRun) {
+      if (!graph_exec_[3]) CL_TRY(capture(3, &graph_exec_[3], s));
+      for (; rest >= kGraphRun; rest -= kGraphRun) CL_HIP(hipGraphLaunch(graph_exec_[3], s));
+    }
+    for (; rest > 0; rest--) CL_HIP(hipGraphLaunch(graph_exec_[0], s));
     CL_HIP(hipGraphLaunch(graph_exec_[1], s));
     return CLARITY_SUCCESS;
   }
 
-  // kind 0: chained body, 1: last body, 2: x_fwd + unchained body
+  // kind 0: chained body, 1: last body, 2: x_fwd + unchained body, 3:
+  // kGraphRun chained bodies
   ClarityResult_t capture(int kind, hipGraphExec_t *out, hipStream_t s) {
     hipGraph_t graph = nullptr;
     CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     ClarityResult_t r = (kind == 2) ? run(KX_FWD, s) : CLARITY_SUCCESS;
-    if (r == CLARITY_SUCCESS) r = enqueue_body(kind == 0, s);
+    for (int i = 0; i < (kind == 3 ? kGraphRun : 1) && r == CLARITY_SUCCESS; i++)
+      r = enqueue_body(kind == 0 || kind == 3, s);
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (r != CLARITY_SUCCESS) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -2599,7 +2609,8 @@ class FastRL : public RLEngine {
   Geom g_{};
   Tune tu_{};
   bool use_graph_ = std::getenv("CLARITY_NO_GRAPH") == nullptr;
-  hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};  // see capture()
+  hipGraphExec_t graph_exec_[4] = {nullptr, nullptr, nullptr, nullptr};  // see capture()
+  static constexpr int kGraphRun = 8;  // chained bodies per graph of kind 3
   Bufs b_{};
   int nparts_ = 0;
   DevBuf obs_, u_, S_, K_, parts_, scal_, pad_;
@@ -2720,7 +2731,14 @@ class FastGroup : public RLGroup {
     }
     for (int k = 0; k < 2; k++)
       if (!graph_[k]) CL_TRY(capture(k == 0, &graph_[k], s));
-    for (int it = 0; it + 1 < n; it++) CL_HIP(hipGraphLaunch(graph_[0], s));
+    // runs of kGraphRun chained bodies per graph (no idle gap between the
+    // graph's kernels; ~9 us between successive graphs)
+    int rest = n - 1;
+    if (rest >= kGraphRun) {
+      if (!graph_[2]) CL_TRY(capture(true, &graph_[2], s, kGraphRun));
+      for (; rest >= kGraphRun; rest -= kGraphRun) CL_HIP(hipGraphLaunch(graph_[2], s));
+    }
+    for (; rest > 0; rest--) CL_HIP(hipGraphLaunch(graph_[0], s));
     CL_HIP(hipGraphLaunch(graph_[1], s));
     return CLARITY_SUCCESS;
   }
@@ -2756,10 +2774,11 @@ class FastGroup : public RLGroup {
     return launch_check();
   }
 
-  ClarityResult_t capture(bool chained, hipGraphExec_t *out, hipStream_t s) {
+  ClarityResult_t capture(bool chained, hipGraphExec_t *out, hipStream_t s, int bodies = 1) {
     hipGraph_t graph = nullptr;
     CL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const ClarityResult_t r = body(chained, s);
+    ClarityResult_t r = CLARITY_SUCCESS;
+    for (int i = 0; i < bodies && r == CLARITY_SUCCESS; i++) r = body(chained, s);
     const hipError_t e = hipStreamEndCapture(s, &graph);
     if (r != CLARITY_SUCCESS) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -2778,7 +2797,8 @@ class FastGroup : public RLGroup {
   std::vector<FastRL *> e_;
   std::vector<std::unique_ptr<Launch>> launches_[KNUM][2];
   DevBuf fin_;
-  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_[3] = {nullptr, nullptr, nullptr};  // chained, last, kGraphRun chained
+  static constexpr int kGraphRun = 8;
 };
 
 }  // namespace
