@@ -325,6 +325,10 @@ __device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi
         const int xo = min(x0 + mt * 16 + m, v.W - 1), zo = min(z0 + o, v.D - 1);
         ov[o][mt] = h.obs[((long long)zo * v.H + y) * v.W + xo];
       }
+    // each lane's 4-channel share of the head, then the two lane exchanges
+    // for all NP x 4 values at once (one LDS round trip per stage, not 2
+    // per value)
+    float pp[NP][4];
 #pragma unroll
     for (int o = 0; o < NP; o++)
 #pragma unroll
@@ -333,12 +337,27 @@ __device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi
         const float d1 = bf2f(f2bf(relu(acc[o][mt][1] + e.bb.y)));
         const float d2 = bf2f(f2bf(relu(acc[o][mt][2] + e.bb.z)));
         const float d3 = bf2f(f2bf(relu(acc[o][mt][3] + e.bb.w)));
-        float p = e.ww.x * d0 + e.ww.y * d1 + e.ww.z * d2 + e.ww.w * d3;
-        p += __shfl_xor(p, 16);
-        p += __shfl_xor(p, 32);
+        pp[o][mt] = e.ww.x * d0 + e.ww.y * d1 + e.ww.z * d2 + e.ww.w * d3;
+      }
+#pragma unroll
+    for (int sh = 16; sh <= 32; sh *= 2) {
+      float q[NP][4];
+#pragma unroll
+      for (int o = 0; o < NP; o++)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) q[o][mt] = __shfl_xor(pp[o][mt], sh);
+#pragma unroll
+      for (int o = 0; o < NP; o++)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) pp[o][mt] += q[o][mt];
+    }
+#pragma unroll
+    for (int o = 0; o < NP; o++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
         const int xo = x0 + mt * 16 + m;
         if (kq == 0 && xo < v.W && z0 + o < v.D) {
-          float r = ov[o][mt] * e.im + (e.hb + p);
+          float r = ov[o][mt] * e.im + (e.hb + pp[o][mt]);
           if (h.rescale) r = (r > 0.f ? r : 0.f) / e.im;
           h.out[((long long)(z0 + o) * v.H + y) * v.W + xo] = r;
         }
