@@ -351,7 +351,9 @@ def test_batch_uneven_volumes(clarity_gpu, path):
 def test_graph_replay_matches_eager(clarity_gpu):
     """An iteration captured as a HIP graph and replayed gives bit-identical
     results to eager launches, replay after replay, with eager work (the
-    estimate copy-out and a host sync) between the replays."""
+    estimate copy-out and a host sync) between the replays; iteration counts
+    below, at and past the graphs of 8 chained iterations (3, 9 = 8 + last,
+    17 = 2 x 8 + last, 8 = 7 single + last)."""
     api = clarity_gpu
     case = synth.make_workload("beads-64x64x32")
     os.environ["CLARITY_NO_GRAPH"] = "1"
@@ -362,11 +364,11 @@ def test_graph_replay_matches_eager(clarity_gpu):
     graph = api.RLPlan(case.obs.shape, case.psf.shape)
     for p in (eager, graph):
         p.set_inputs(case.obs, case.psf)
-    for k in range(4):
-        eager.iterate(3)
-        graph.iterate(3)
+    for k, n in enumerate((3, 9, 17, 8)):
+        eager.iterate(n)
+        graph.iterate(n)
         a, b = eager.estimate(), graph.estimate()
-        assert np.array_equal(a, b), k
+        assert np.array_equal(a, b), (k, n)
         assert eager.scale() == graph.scale()
     eager.close()
     graph.close()
