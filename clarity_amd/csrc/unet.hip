@@ -497,8 +497,6 @@ __global__ void __launch_bounds__(256)
       if (o < 0 || o >= kIB) continue;                                                    \
       if constexpr (WLDS)                                                                 \
         wa[B][dz + 1] = __builtin_bit_cast(bf16x8, wl[((c * 3 + dz + 1) * 5 + (P)) * 64 + lane]); \
-      else                                                                                \
-        wa[B][dz + 1] = w[(dz + 1) * 5 + (P)];                                            \
     }                                                                                     \
   }
     CLARITY_IP_READ_PAIR(0, 0)
@@ -512,8 +510,8 @@ __global__ void __launch_bounds__(256)
         if (o < 0 || o >= kIB) continue;
 #pragma unroll
         for (int mt = 0; mt < 4; mt++)
-          acc[o][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[p & 1][dz + 1], fa[p & 1][mt],
-                                                               acc[o][mt], 0, 0, 0);
+          acc[o][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              WLDS ? wa[p & 1][dz + 1] : w[(dz + 1) * 5 + p], fa[p & 1][mt], acc[o][mt], 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
