@@ -411,11 +411,14 @@ constexpr int kIB = 4;        // output planes per block
 constexpr int kIPDepth = 2;   // units whose loads are in flight
 constexpr bool kIPWLds = false;  // weights in LDS also for one chunk
 
-template <int CIN, int CA, bool UP, bool HEAD>
+// POOL: also the 2x2x2 average pooling of the output (k_pool's sums, in
+// its order), into P at half resolution (needs v.H % 4 == 0: every row of a
+// tile is inside the volume).
+template <int CIN, int CA, bool UP, bool HEAD, bool POOL = false>
 __global__ void __launch_bounds__(256)
     k_conv_ip(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
-              bf16 *__restrict__ Y, Vol v, HeadArgs h) {
+              bf16 *__restrict__ Y, Vol v, HeadArgs h, bf16 *__restrict__ P = nullptr) {
   static_assert(CIN % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)), "channel chunks of 16");
   constexpr int COUT = 16, NCH = CIN / 16, NU = (kIB + 2) * NCH;
   constexpr bool WLDS = NCH > 1 || kIPWLds;
@@ -521,8 +524,52 @@ __global__ void __launch_bounds__(256)
   // row/plane bounds branch, would make the wait for a unit's loads also
   // wait for the loads issued after it (vmcnt counts loads and stores in
   // order, and the count past a branch is taken as the smaller one)
-  if (y >= v.H) return;
-  epi16_store<kIB, HEAD>(acc, ep, Y, h, v, x0, y, z0, m, kq);
+  if constexpr (!POOL) {
+    if (y >= v.H) return;
+    epi16_store<kIB, HEAD>(acc, ep, Y, h, v, x0, y, z0, m, kq);
+  } else {
+    static_assert(!HEAD && kIB % 2 == 0, "pooling of the stored activation");
+    epi16_store<kIB, false>(acc, ep, Y, h, v, x0, y, z0, m, kq);
+    // the tile's bfloat16 outputs, two planes at a time, staged in LDS as
+    // [plane][row][x][channel quad] (8 B each; the unit buffers are free
+    // now), then one pooled voxel's channel quad per thread
+    uint2 *st = reinterpret_cast<uint2 *>(buf);
+    static_assert(2 * kYT * kXT * 4 * 8 <= (int)sizeof(buf), "staging fits");
+    const int W2 = v.W >> 1, H2 = v.H >> 1;
+    const int quad = threadIdx.x & 3, vox = threadIdx.x >> 2;  // 64 pooled voxels
+    const int px = vox & 31, py = vox >> 5;
+#pragma unroll
+    for (int pp = 0; pp < kIB / 2; pp++) {
+      __syncthreads();  // the buffers' last readers are done
+#pragma unroll
+      for (int oo = 0; oo < 2; oo++)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+          const f32x4 &a = acc[2 * pp + oo][mt];
+          uint2 q;
+          q.x = (unsigned)f2bf(relu(a[0] + ep.bb.x)) | ((unsigned)f2bf(relu(a[1] + ep.bb.y)) << 16);
+          q.y = (unsigned)f2bf(relu(a[2] + ep.bb.z)) | ((unsigned)f2bf(relu(a[3] + ep.bb.w)) << 16);
+          st[((oo * kYT + wave) * kXT + mt * 16 + m) * 4 + kq] = q;
+        }
+      __syncthreads();
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint2 q = st[(((k >> 2) * kYT + 2 * py + ((k >> 1) & 1)) * kXT + 2 * px + (k & 1)) * 4 + quad];
+        sum[0] += bf2f((bf16)(q.x & 0xffffu));
+        sum[1] += bf2f((bf16)(q.x >> 16));
+        sum[2] += bf2f((bf16)(q.y & 0xffffu));
+        sum[3] += bf2f((bf16)(q.y >> 16));
+      }
+      const int gx = (x0 >> 1) + px, gy = (y0 >> 1) + py, gz = (z0 >> 1) + pp;
+      if (gx < W2 && 2 * gz < v.D) {
+        uint2 o;
+        o.x = (unsigned)f2bf(0.125f * sum[0]) | ((unsigned)f2bf(0.125f * sum[1]) << 16);
+        o.y = (unsigned)f2bf(0.125f * sum[2]) | ((unsigned)f2bf(0.125f * sum[3]) << 16);
+        *reinterpret_cast<uint2 *>(P + (((long long)gz * H2 + gy) * W2 + gx) * 16 + 4 * quad) = o;
+      }
+    }
+  }
 }
 
 constexpr int kCoDepth = 1;  // units whose loads are in flight (k_conv_co)
@@ -1157,6 +1204,13 @@ void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf
                        dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
                        X, S, Wf, b, Y, v);
 }
+// the second convolution (16 -> 16) with the first pooling in its epilogue
+void conv_fwd_pool(const bf16 *X, const bf16 *Wf, const float *b, bf16 *Y, bf16 *P, Vol v,
+                   hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_ip<16, 0, false, false, true>),
+                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + kIB - 1) / kIB),
+                     dim3(256), 0, s, X, nullptr, Wf, b, Y, v, HeadArgs{}, P);
+}
 // the last convolution (16 -> 16) with the head fused into its epilogue
 void conv_fwd_head(const bf16 *X, const bf16 *Wf, const float *b, const HeadArgs &h, Vol v,
                    hipStream_t s) {
@@ -1349,9 +1403,7 @@ class UNet {
     src[8] = (bf16 *)cat1_.p;
     src[9] = (bf16 *)act_[8].p;
     for (int i = 0; i < 10; i++) {
-      if (i == 2)
-        hipLaunchKernelGGL(k_pool<16>, dim3(grid_for(v1.n() * 2)), dim3(256), 0, s,
-                           (bf16 *)act_[1].p, (bf16 *)pool1_.p, v);
+      // (the first pooling runs in the epilogue of the layer before, i = 1)
       if (i == 4)
         hipLaunchKernelGGL(k_pool<32>, dim3(grid_for(v2.n() * 4)), dim3(256), 0, s,
                            (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1);
@@ -1359,6 +1411,10 @@ class UNet {
       const float *b = p_.f() + off_[10 + i];
       const bf16 *Wf = (bf16 *)wf_[i].p;
       const Vol vv = lv[kLevel[i]];
+      if (i == 1) {
+        conv_fwd_pool(src[1], Wf, b, Y, (bf16 *)pool1_.p, v, s);
+        continue;
+      }
       if (i == 0 && head_out) {
         hipLaunchKernelGGL((k_conv_in<kInIB>),
                            dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT,
