@@ -41,6 +41,14 @@ __device__ __forceinline__ bf16 f2bf(float f) {
   return (bf16)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((unsigned)h << 16); }
+// f2bf(a) | f2bf(b) << 16 in one instruction (v_cvt_pk_bf16_f32, round to
+// nearest even: the same bits as f2bf for every non-NaN value; the
+// activations it rounds are ReLU outputs and sums of them, never NaN)
+typedef float f2v_t __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pk_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f2v_t){a, b}, b2v_t));
+}
 // max(r, 0) with NaN -> +0 and -0 -> +0 (a stored activation is never -0)
 __device__ __forceinline__ float relu(float r) { return r > 0.f ? r : 0.f; }
 
@@ -107,8 +115,8 @@ __device__ __forceinline__ void conv_epilogue(const f32x4 (&acc)[4][COUT / 16],
         const float r0 = relu(acc[mt][ct][0] + bb.x), r1 = relu(acc[mt][ct][1] + bb.y);
         const float r2 = relu(acc[mt][ct][2] + bb.z), r3 = relu(acc[mt][ct][3] + bb.w);
         uint2 o;
-        o.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
-        o.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+        o.x = pk_bf16(r0, r1);
+        o.y = pk_bf16(r2, r3);
         *reinterpret_cast<uint2 *>(Y + (rowbase + xo) * COUT + co) = o;
       }
     }
@@ -333,10 +341,10 @@ __device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi
     for (int o = 0; o < NP; o++)
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        const float d0 = bf2f(f2bf(relu(acc[o][mt][0] + e.bb.x)));
-        const float d1 = bf2f(f2bf(relu(acc[o][mt][1] + e.bb.y)));
-        const float d2 = bf2f(f2bf(relu(acc[o][mt][2] + e.bb.z)));
-        const float d3 = bf2f(f2bf(relu(acc[o][mt][3] + e.bb.w)));
+        const unsigned q01 = pk_bf16(relu(acc[o][mt][0] + e.bb.x), relu(acc[o][mt][1] + e.bb.y));
+        const unsigned q23 = pk_bf16(relu(acc[o][mt][2] + e.bb.z), relu(acc[o][mt][3] + e.bb.w));
+        const float d0 = __uint_as_float(q01 << 16), d1 = __uint_as_float(q01 & 0xffff0000u);
+        const float d2 = __uint_as_float(q23 << 16), d3 = __uint_as_float(q23 & 0xffff0000u);
         pp[o][mt] = e.ww.x * d0 + e.ww.y * d1 + e.ww.z * d2 + e.ww.w * d3;
       }
 #pragma unroll
@@ -372,8 +380,8 @@ __device__ __forceinline__ void epi16_store(const f32x4 (&acc)[NP][4], const Epi
           const float r0 = relu(acc[o][mt][0] + e.bb.x), r1 = relu(acc[o][mt][1] + e.bb.y);
           const float r2 = relu(acc[o][mt][2] + e.bb.z), r3 = relu(acc[o][mt][3] + e.bb.w);
           uint2 q;
-          q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
-          q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+          q.x = pk_bf16(r0, r1);
+          q.y = pk_bf16(r2, r3);
           *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * 16 +
                                      4 * kq) = q;
         }
@@ -547,8 +555,8 @@ __global__ void __launch_bounds__(256)
         for (int mt = 0; mt < 4; mt++) {
           const f32x4 &a = acc[2 * pp + oo][mt];
           uint2 q;
-          q.x = (unsigned)f2bf(relu(a[0] + ep.bb.x)) | ((unsigned)f2bf(relu(a[1] + ep.bb.y)) << 16);
-          q.y = (unsigned)f2bf(relu(a[2] + ep.bb.z)) | ((unsigned)f2bf(relu(a[3] + ep.bb.w)) << 16);
+          q.x = pk_bf16(relu(a[0] + ep.bb.x), relu(a[1] + ep.bb.y));
+          q.y = pk_bf16(relu(a[2] + ep.bb.z), relu(a[3] + ep.bb.w));
           st[((oo * kYT + wave) * kXT + mt * 16 + m) * 4 + kq] = q;
         }
       __syncthreads();
@@ -564,8 +572,8 @@ __global__ void __launch_bounds__(256)
       const int gx = (x0 >> 1) + px, gy = (y0 >> 1) + py, gz = (z0 >> 1) + pp;
       if (gx < W2 && 2 * gz < v.D) {
         uint2 o;
-        o.x = (unsigned)f2bf(0.125f * sum[0]) | ((unsigned)f2bf(0.125f * sum[1]) << 16);
-        o.y = (unsigned)f2bf(0.125f * sum[2]) | ((unsigned)f2bf(0.125f * sum[3]) << 16);
+        o.x = pk_bf16(0.125f * sum[0], 0.125f * sum[1]);
+        o.y = pk_bf16(0.125f * sum[2], 0.125f * sum[3]);
         *reinterpret_cast<uint2 *>(P + (((long long)gz * H2 + gy) * W2 + gx) * 16 + 4 * quad) = o;
       }
     }
@@ -685,8 +693,8 @@ __global__ void __launch_bounds__(256)
           const float r0 = relu(acc[o][ct][mt][0] + bb[ct].x), r1 = relu(acc[o][ct][mt][1] + bb[ct].y);
           const float r2 = relu(acc[o][ct][mt][2] + bb[ct].z), r3 = relu(acc[o][ct][mt][3] + bb[ct].w);
           uint2 q;
-          q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
-          q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+          q.x = pk_bf16(r0, r1);
+          q.y = pk_bf16(r2, r3);
           *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * COUT +
                                      ct * 16 + 4 * kq) = q;
         }
@@ -716,8 +724,8 @@ __global__ void __launch_bounds__(256)
   const int m = lane & 15, kq = lane >> 4;
   const int x0 = blockIdx.x * kXT, y0 = blockIdx.y * kYT, z0 = blockIdx.z * IB;
   const float im = inv_mean[0];
-  const unsigned w0hi = (unsigned)f2bf(sxy * 0.25f) << 16;  // channel 1
-  const unsigned w1 = (unsigned)f2bf(sz * 0.125f);          // channel 2 (3: zero)
+  // channel 1 (sigma_xy/4) is packed with channel 0 below
+  const unsigned w1 = (unsigned)f2bf(sz * 0.125f);  // channel 2 (3: zero)
   const float4 bb = *reinterpret_cast<const float4 *>(bias + 4 * kq);
   // weights of the four K = 32 groups (taps 8p + 2g, 8p + 2g + 1)
   bf16x8 w[4];
@@ -749,7 +757,7 @@ __global__ void __launch_bounds__(256)
   for (int j = 0; j < NI; j++) {
     const int i = threadIdx.x + 256 * j;
     const unsigned mk = inb[j] ? 0xffffffffu : 0u;
-    if (i < NV) tile[i] = make_uint2(((unsigned)f2bf(ov[j] * im) | w0hi) & mk, w1 & mk);
+    if (i < NV) tile[i] = make_uint2(pk_bf16(ov[j] * im, sxy * 0.25f) & mk, w1 & mk);
   }
   __syncthreads();
   const int y = y0 + wave;
@@ -783,8 +791,8 @@ __global__ void __launch_bounds__(256)
         const float r0 = relu(acc[mt][0] + bb.x), r1 = relu(acc[mt][1] + bb.y);
         const float r2 = relu(acc[mt][2] + bb.z), r3 = relu(acc[mt][3] + bb.w);
         uint2 q;
-        q.x = (unsigned)f2bf(r0) | ((unsigned)f2bf(r1) << 16);
-        q.y = (unsigned)f2bf(r2) | ((unsigned)f2bf(r3) << 16);
+        q.x = pk_bf16(r0, r1);
+        q.y = pk_bf16(r2, r3);
         *reinterpret_cast<uint2 *>(Y + (((long long)(z0 + o) * v.H + y) * v.W + xo) * COUT +
                                    4 * kq) = q;
       }
@@ -1001,7 +1009,7 @@ __global__ void __launch_bounds__(256) k_pool(const bf16 *__restrict__ X, bf16 *
     unsigned o4[4];
 #pragma unroll
     for (int j = 0; j < 4; j++)
-      o4[j] = (unsigned)f2bf(0.125f * sum[2 * j]) | ((unsigned)f2bf(0.125f * sum[2 * j + 1]) << 16);
+      o4[j] = pk_bf16(0.125f * sum[2 * j], 0.125f * sum[2 * j + 1]);
     *reinterpret_cast<uint4 *>(Y + (long long)p * C + 8 * oc) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
   }
 }
