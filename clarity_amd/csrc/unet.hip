@@ -225,6 +225,12 @@ __global__ void __launch_bounds__(256)
 // plane-by-plane convolution below: two LDS buffers of [octet][voxel].
 constexpr int kPV = kTX * kTY;               // 396 voxels per staged plane
 constexpr int kPL = (2 * kPV + 255) / 256;   // 16-byte loads per thread and plane
+// 4 voxels (64 B) between the two octet planes in LDS: unpadded, the
+// 6336-byte plane stride sets the two octets' fragment reads on the same
+// banks (0.31-0.35 conflict cycles per active LDS cycle, 0.12-0.14 padded;
+// 8 voxels map like none, 1, 2 or 6 gain nothing)
+constexpr int kPad = 4;
+constexpr int kPS = kPV + kPad;              // LDS stride of an octet plane
 
 // The thread's staging slots of a plane: byte offsets within a plane (x, y
 // clamped into the volume; for an UP input also those into the coarse
@@ -286,7 +292,7 @@ __device__ __forceinline__ void zm_store(const uint4 (&r)[kPL], uint4 *ring, int
 #pragma unroll
   for (int j = 0; j < kPL; j++) {
     const int i = threadIdx.x + 256 * j;
-    if (i < 2 * kPV) ring[(slot * 2 + (i & 1)) * kPV + (i >> 1)] = r[j];
+    if (i < 2 * kPV) ring[(slot * 2 + (i & 1)) * kPS + (i >> 1)] = r[j];
   }
 }
 
@@ -430,7 +436,7 @@ __global__ void __launch_bounds__(256)
   static_assert(CIN % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)), "channel chunks of 16");
   constexpr int COUT = 16, NCH = CIN / 16, NU = (kIB + 2) * NCH;
   constexpr bool WLDS = NCH > 1 || kIPWLds;
-  __shared__ uint4 buf[2 * 2 * kPV];
+  __shared__ uint4 buf[2 * 2 * kPS];
   __shared__ uint4 wl[WLDS ? NCH * 15 * 64 : 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
@@ -469,7 +475,7 @@ __global__ void __launch_bounds__(256)
   for (int p = 0; p < 5; p++) {
     const int t0 = 2 * p, t1 = 2 * p + 1 < 9 ? 2 * p + 1 : 8;
     const int o0 = (t0 / 3) * kTX + t0 % 3, o1 = (t1 / 3) * kTX + t1 % 3;
-    foff[p] = oct * kPV + wave * kTX + m + (e ? o1 : o0);
+    foff[p] = oct * kPS + wave * kTX + m + (e ? o1 : o0);
   }
   const ZmSlots t = zm_slots<CIN, CA, UP>(v, x0, y0);
   const int y = y0 + wave;
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) acc[s][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    const uint4 *bb = buf + (u & 1) * 2 * kPV;
+    const uint4 *bb = buf + (u & 1) * 2 * kPS;
     // software pipeline over the five pairs: pair p + 1's LDS reads are
     // issued before pair p's MFMAs (scheduling barriers keep that order;
     // left alone, the compiler reads each fragment just before its MFMA
@@ -597,7 +603,7 @@ __global__ void __launch_bounds__(256)
                 "channel chunks of 16");
   constexpr int NCT = COUT / 16, NCH = CIN / 16, NS = IB + 2, NU = NCH * NS;
   constexpr int NW = NCT * 15 * 64;  // weight fragments of a chunk
-  __shared__ uint4 buf[2 * 2 * kPV];
+  __shared__ uint4 buf[2 * 2 * kPS];
   __shared__ uint4 wl[NW];  // [dz][pair][ct][lane]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
@@ -608,7 +614,7 @@ __global__ void __launch_bounds__(256)
   for (int p = 0; p < 5; p++) {
     const int t0 = 2 * p, t1 = 2 * p + 1 < 9 ? 2 * p + 1 : 8;
     const int o0 = (t0 / 3) * kTX + t0 % 3, o1 = (t1 / 3) * kTX + t1 % 3;
-    foff[p] = oct * kPV + wave * kTX + m + (e ? o1 : o0);
+    foff[p] = oct * kPS + wave * kTX + m + (e ? o1 : o0);
   }
   const ZmSlots t = zm_slots<CIN, CA, UP>(v, x0, y0);
   const int y = y0 + wave;
@@ -648,7 +654,7 @@ __global__ void __launch_bounds__(256)
     if constexpr (u + kCoDepth < NU)
       zm_load<CIN, CA, UP>(r[u % kCoDepth], t, X, S, v, z0 - 1 + (u + kCoDepth) % NS,
                            16 * ((u + kCoDepth) / NS));
-    const uint4 *bq = buf + (u & 1) * 2 * kPV;
+    const uint4 *bq = buf + (u & 1) * 2 * kPS;
     bf16x8 fa[2][4], wa[2][3][NCT];
 #define CLARITY_CO_READ_PAIR(P, B)                                                        \
   {                                                                                       \
