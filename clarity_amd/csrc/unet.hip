@@ -545,8 +545,8 @@ __global__ void __launch_bounds__(256)
     static_assert(!HEAD && kIB % 2 == 0, "pooling of the stored activation");
     epi16_store<kIB, false>(acc, ep, Y, h, v, x0, y, z0, m, kq);
     // the tile's bfloat16 outputs, two planes at a time, staged in LDS as
-    // [plane][row][x][channel quad] (8 B each; the unit buffers are free
-    // now), then one pooled voxel's channel quad per thread
+    // [plane][row][x parity][x / 2][channel quad] (8 B each; the unit
+    // buffers are free now), then one pooled voxel's channel quad per thread
     uint2 *st = reinterpret_cast<uint2 *>(buf);
     static_assert(2 * kYT * kXT * 4 * 8 <= (int)sizeof(buf), "staging fits");
     const int W2 = v.W >> 1, H2 = v.H >> 1;
@@ -563,13 +563,16 @@ __global__ void __launch_bounds__(256)
           uint2 q;
           q.x = pk_bf16(relu(a[0] + ep.bb.x), relu(a[1] + ep.bb.y));
           q.y = pk_bf16(relu(a[2] + ep.bb.z), relu(a[3] + ep.bb.w));
-          st[((oo * kYT + wave) * kXT + mt * 16 + m) * 4 + kq] = q;
+          // even x in the first half of a row, odd x in the second: the
+          // pooling's reads of consecutive pooled voxels are then contiguous
+          const int xs = mt * 16 + m;
+          st[((oo * kYT + wave) * kXT + (xs & 1) * (kXT / 2) + (xs >> 1)) * 4 + kq] = q;
         }
       __syncthreads();
       float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const uint2 q = st[(((k >> 2) * kYT + 2 * py + ((k >> 1) & 1)) * kXT + 2 * px + (k & 1)) * 4 + quad];
+        const uint2 q = st[(((k >> 2) * kYT + 2 * py + ((k >> 1) & 1)) * kXT + (k & 1) * (kXT / 2) + px) * 4 + quad];
         sum[0] += bf2f((bf16)(q.x & 0xffffu));
         sum[1] += bf2f((bf16)(q.x >> 16));
         sum[2] += bf2f((bf16)(q.y & 0xffffu));
