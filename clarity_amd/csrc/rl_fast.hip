@@ -453,9 +453,12 @@ __device__ __forceinline__ int padded(int pos) {
 }
 template <int N, int PS>
 constexpr int kPadded = N + (N >> PS);
-// pad interval for E elements per thread (the first stage's radix)
+// pad interval for E elements per thread (the first stage's radix): a slot
+// every 16 elements at 16 per thread, every 4 at 12 per thread (every 8:
+// x_mid 107.5 and x_last_fwd 148.6 us at 512x512x128 against 105.2 and
+// 145.4; every 16 or 32 slower still)
 template <int E>
-constexpr int kPadShift = (E == 16) ? 4 : 3;
+constexpr int kPadShift = (E == 16) ? 4 : 2;
 
 // Row q of an x-pass tile: padded, and rows kRowStride apart, an odd number
 // of slots, so that a column access (same pos, consecutive rows) also hits
@@ -638,10 +641,11 @@ template <> struct Plan<384, 12> { using Rad = Radices<6, 4, 4, 4>; };
 template <> struct Plan<768, 12> { using Rad = Radices<12, 4, 4, 4>; };
 
 // Radix plans of the x passes, whose rows sit in LDS with one pad slot per
-// 8 elements: a stage writes its outputs at positions base + r * P (span P
-// so far, r < R), which stay affine in r after padding -- one address
-// register plus immediate offsets -- only when P is a multiple of 8 or P *
-// R stays inside one pad period (P = 1 with R <= 4; P = 4 with R = 4).  A
+// 4 elements (12 per thread; 16 at 16 per thread): a stage writes its
+// outputs at positions base + r * P (span P so far, r < R), which stay
+// affine in r after padding -- one address register plus immediate
+// offsets -- only when P is a multiple of the pad period or P * R stays
+// inside one pad period (P = 1 with R <= 4).  A
 // radix 6 or 12 stage first breaks that for the first two stages (3-4
 // integer instructions per element written); taking it last instead costs
 // its R - 1 twiddle products per butterfly and leaves every write affine:
