@@ -456,7 +456,8 @@ constexpr int kPadded = N + (N >> PS);
 // pad interval for E elements per thread (the first stage's radix): a slot
 // every 16 elements at 16 per thread, every 4 at 12 per thread (every 8:
 // x_mid 107.5 and x_last_fwd 148.6 us at 512x512x128 against 105.2 and
-// 145.4; every 16 or 32 slower still)
+// 145.4; every 16 or 32 slower still; at 16 per thread every 8 or 4 took
+// the 512-point x_mid from 133 to 204-215 us)
 template <int E>
 constexpr int kPadShift = (E == 16) ? 4 : 2;
 
