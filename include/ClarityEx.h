@@ -159,8 +159,9 @@ C_FUNC_DEF ClarityResult_t ClarityEx_RLPlanWorkDims(ClarityEx_RLPlan plan,
  * connections; bfloat16 weights on MFMA with float32 accumulation) that maps
  * an observation and its kernel widths to an estimate of the 200-iteration
  * RL result, used to seed a short RL run (ClarityEx_RLPlanReset).  Volume
- * dims must be multiples of 4.  Parameters are float32 (He-initialised from
- * `seed`). */
+ * dims must be multiples of 4 and an x-y plane under 2^32 / 96 voxels
+ * (CLARITY_INVALID_ARGUMENT otherwise).  Parameters are float32
+ * (He-initialised from `seed`). */
 typedef struct ClarityEx_UNet_t *ClarityEx_UNet;
 
 C_FUNC_DEF ClarityResult_t ClarityEx_UNetCreate(ClarityEx_UNet *net, unsigned seed);
