@@ -425,9 +425,10 @@ constexpr int kIB = 4;        // output planes per block
 constexpr int kIPDepth = 2;   // units whose loads are in flight
 constexpr bool kIPWLds = false;  // weights in LDS also for one chunk
 
-// POOL: also the 2x2x2 average pooling of the output (k_pool's sums, in
-// its order), into P at half resolution (needs v.H % 4 == 0: every row of a
-// tile is inside the volume).
+// POOL: also the 2x2x2 average pooling of the stored bfloat16 output into P
+// at half resolution (each channel sums its eight inputs in the order
+// k = 0..7 of the z, y, x offset bits, then scales by 1/8); needs
+// v.H % 4 == 0, so every row of a tile is inside the volume.
 template <int CIN, int CA, bool UP, bool HEAD, bool POOL = false>
 __global__ void __launch_bounds__(256)
     k_conv_ip(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
@@ -597,11 +598,14 @@ constexpr int kCoDepth = 1;  // units whose loads are in flight (k_conv_co)
 // and every output plane of the block keeps its accumulators to the end.
 // Per output voxel the sum runs over chunks, then input planes z-1, z, z+1,
 // then pairs.
-template <int CIN, int COUT, int CA, bool UP, int IB>
+// POOL (IB = 2): also the 2x2x2 average pooling of the output into P, as
+// k_conv_ip's (the level's sizes are even, so a pooled voxel's rows and
+// planes are all inside the volume or all outside).
+template <int CIN, int COUT, int CA, bool UP, int IB, bool POOL = false>
 __global__ void __launch_bounds__(256)
     k_conv_co(const bf16 *__restrict__ X, const bf16 *__restrict__ S,
               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
-              bf16 *__restrict__ Y, Vol v) {
+              bf16 *__restrict__ Y, Vol v, bf16 *__restrict__ P = nullptr) {
   static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
                 "channel chunks of 16");
   constexpr int NCT = COUT / 16, NCH = CIN / 16, NS = IB + 2, NU = NCH * NS;
@@ -690,7 +694,7 @@ __global__ void __launch_bounds__(256)
     }
 #undef CLARITY_CO_READ_PAIR
   });
-  if (y >= v.H) return;
+  if (!POOL && y >= v.H) return;
 #pragma unroll
   for (int o = 0; o < IB; o++)
 #pragma unroll
@@ -698,7 +702,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
         const int xo = x0 + mt * 16 + m;
-        if (xo < v.W && z0 + o < v.D) {
+        if (xo < v.W && z0 + o < v.D && y < v.H) {
           const float r0 = relu(acc[o][ct][mt][0] + bb[ct].x), r1 = relu(acc[o][ct][mt][1] + bb[ct].y);
           const float r2 = relu(acc[o][ct][mt][2] + bb[ct].z), r3 = relu(acc[o][ct][mt][3] + bb[ct].w);
           uint2 q;
@@ -708,6 +712,50 @@ __global__ void __launch_bounds__(256)
                                      ct * 16 + 4 * kq) = q;
         }
       }
+  if constexpr (POOL) {
+    static_assert(IB == 2, "one pooled plane per block");
+    // per 16-channel tile: the two planes' bfloat16 outputs staged in LDS as
+    // [plane][row][x parity][x / 2][channel quad], then one pooled voxel's
+    // channel quad per thread, summed in k = 0..7 order
+    uint2 *st = reinterpret_cast<uint2 *>(buf);
+    static_assert(2 * kYT * kXT * 4 * 8 <= (int)sizeof(buf), "staging fits");
+    const int W2 = v.W >> 1, H2 = v.H >> 1;
+    const int quad = threadIdx.x & 3, vox = threadIdx.x >> 2;
+    const int px = vox & 31, py = vox >> 5;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ct++) {
+      __syncthreads();  // the buffers' last readers are done
+#pragma unroll
+      for (int o = 0; o < 2; o++)
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+          const f32x4 &a = acc[o][ct][mt];
+          uint2 q;
+          q.x = pk_bf16(relu(a[0] + bb[ct].x), relu(a[1] + bb[ct].y));
+          q.y = pk_bf16(relu(a[2] + bb[ct].z), relu(a[3] + bb[ct].w));
+          const int xs = mt * 16 + m;
+          st[((o * kYT + wave) * kXT + (xs & 1) * (kXT / 2) + (xs >> 1)) * 4 + kq] = q;
+        }
+      __syncthreads();
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint2 q = st[(((k >> 2) * kYT + 2 * py + ((k >> 1) & 1)) * kXT + (k & 1) * (kXT / 2) + px) * 4 + quad];
+        sum[0] += bf2f((bf16)(q.x & 0xffffu));
+        sum[1] += bf2f((bf16)(q.x >> 16));
+        sum[2] += bf2f((bf16)(q.y & 0xffffu));
+        sum[3] += bf2f((bf16)(q.y >> 16));
+      }
+      const int gx = (x0 >> 1) + px, gy = (y0 >> 1) + py, gz = z0 >> 1;
+      if (gx < W2 && gy < H2 && 2 * gz < v.D) {
+        uint2 o2;
+        o2.x = pk_bf16(0.125f * sum[0], 0.125f * sum[1]);
+        o2.y = pk_bf16(0.125f * sum[2], 0.125f * sum[3]);
+        *reinterpret_cast<uint2 *>(P + (((long long)gz * H2 + gy) * W2 + gx) * COUT + ct * 16 +
+                                   4 * quad) = o2;
+      }
+    }
+  }
 }
 
 // The first convolution for inference: the network input (obs/mean,
@@ -984,45 +1032,6 @@ __global__ void k_reduce_chunks(const float *__restrict__ P, int nchunks, long l
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (n); \
        i += (long long)gridDim.x * blockDim.x)
 
-// 2x2x2 average pooling; v is the fine volume.  One thread per output voxel
-// and octet of channels: eight 16-byte loads, one 16-byte store; each
-// channel sums its eight inputs in the fixed order k = 0..7 (z, y, x bits).
-// Pooled voxel indices fit in 32 bits (2^32 of them would be a 1 TB input).
-template <int C>
-__global__ void __launch_bounds__(256) k_pool(const bf16 *__restrict__ X, bf16 *__restrict__ Y,
-                                              Vol v) {
-  static_assert(C % 8 == 0, "channel octets");
-  constexpr int NO = C / 8;
-  const unsigned D2 = v.D / 2, H2 = v.H / 2, W2 = v.W / 2;
-  const long long n = (long long)D2 * H2 * W2 * NO;
-  VOX_LOOP(i, n) {
-    const int oc = (int)(i % NO);
-    const unsigned p = (unsigned)(i / NO);
-    const unsigned x = p % W2, q = p / W2;
-    const unsigned y = q % H2, z = q / H2;
-    uint4 in[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const long long zz = 2 * z + (k >> 2), yy = 2 * y + ((k >> 1) & 1), xx = 2 * x + (k & 1);
-      in[k] = *reinterpret_cast<const uint4 *>(X + ((zz * v.H + yy) * v.W + xx) * C + 8 * oc);
-    }
-    float sum[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) sum[e] = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const unsigned w4[4] = {in[k].x, in[k].y, in[k].z, in[k].w};
-#pragma unroll
-      for (int e = 0; e < 8; e++) sum[e] += bf2f((bf16)(w4[e >> 1] >> (16 * (e & 1))));
-    }
-    unsigned o4[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-      o4[j] = pk_bf16(0.125f * sum[2 * j], 0.125f * sum[2 * j + 1]);
-    *reinterpret_cast<uint4 *>(Y + (long long)p * C + 8 * oc) = make_uint4(o4[0], o4[1], o4[2], o4[3]);
-  }
-}
-
 // GX (fine) = GX + pool^T(GP) = GX + GP[parent] / 8
 __global__ void k_pool_bwd_add(const bf16 *__restrict__ GP, bf16 *__restrict__ GX, Vol v,
                                int C) {
@@ -1220,6 +1229,13 @@ void conv_fwd_t(const bf16 *X, const bf16 *S, const bf16 *Wf, const float *b, bf
     hipLaunchKernelGGL((k_conv_fwd<CIN, COUT, CA, UP>),
                        dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, v.D), dim3(256), 0, s,
                        X, S, Wf, b, Y, v);
+}
+// the fourth convolution (32 -> 32) with the second pooling in its epilogue
+void conv_fwd_pool32(const bf16 *X, const bf16 *Wf, const float *b, bf16 *Y, bf16 *P, Vol v,
+                     hipStream_t s) {
+  hipLaunchKernelGGL((k_conv_co<32, 32, 0, false, 2, true>),
+                     dim3((v.W + kXT - 1) / kXT, (v.H + kYT - 1) / kYT, (v.D + 1) / 2),
+                     dim3(256), 0, s, X, nullptr, Wf, b, Y, v, P);
 }
 // the second convolution (16 -> 16) with the first pooling in its epilogue
 void conv_fwd_pool(const bf16 *X, const bf16 *Wf, const float *b, bf16 *Y, bf16 *P, Vol v,
@@ -1420,16 +1436,18 @@ class UNet {
     src[8] = (bf16 *)cat1_.p;
     src[9] = (bf16 *)act_[8].p;
     for (int i = 0; i < 10; i++) {
-      // (the first pooling runs in the epilogue of the layer before, i = 1)
-      if (i == 4)
-        hipLaunchKernelGGL(k_pool<32>, dim3(grid_for(v2.n() * 4)), dim3(256), 0, s,
-                           (bf16 *)act_[3].p, (bf16 *)pool2_.p, v1);
+      // the two poolings run in the epilogues of layers 1 and 3
       bf16 *Y = (bf16 *)act_[i].p;
       const float *b = p_.f() + off_[10 + i];
       const bf16 *Wf = (bf16 *)wf_[i].p;
       const Vol vv = lv[kLevel[i]];
       if (i == 1) {
         conv_fwd_pool(src[1], Wf, b, Y, (bf16 *)pool1_.p, v, s);
+        continue;
+      }
+      if (i == 3) {
+        static_assert(kCoIB == 2, "the fused pooling takes two planes per block");
+        conv_fwd_pool32(src[3], Wf, b, Y, (bf16 *)pool2_.p, v1, s);
         continue;
       }
       if (i == 0 && head_out) {
