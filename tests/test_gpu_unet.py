@@ -13,7 +13,7 @@ output tiles per plane and every channel chunk at each level, including
 the first layer's tap packing and the fused head of the inference
 path), and 12x20x72 puts partial tiles in every dimension at once (a last
 first-layer block of 4 of its 8 planes, 5 row tiles, a second x tile of 8
-voxels, the fused pooling's edge at 36 pooled columns)."""
+voxels, the fused poolings' edges at 36 and 18 pooled columns)."""
 import numpy as np
 import pytest
 
@@ -110,3 +110,29 @@ def test_untrained_head_returns_observation(clarity_gpu):
     # head initialised at ~1e-3: the estimate starts at the observation
     assert rel_l2(out, obs) < 0.05
     n.close()
+
+
+def test_predict_is_deterministic(net):
+    """Two inferences of one volume are bit for bit the same (no atomics
+    or order-dependent sums anywhere in the forward pass, including the
+    fused poolings and the observation's mean)."""
+    obs, _ = _sample((12, 20, 72), 11)
+    a = net.predict(obs, 2.0, 4.0)
+    b = net.predict(obs, 2.0, 4.0)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", [(6, 8, 8), (8, 10, 8), (8, 8, 2), (4, 8192, 8192)])
+def test_predict_rejects_shapes(clarity_gpu, net, shape):
+    """Every side a multiple of 4 (two 2x2x2 poolings), and an x-y plane
+    under 2^32 / 96 voxels (48 bfloat16 channels of a plane addressed with
+    32-bit byte offsets): 4 x 8192 x 8192 has 6.4e9 bytes per plane.  The
+    checks run before any allocation or copy (the large input is zero pages
+    that are never touched)."""
+    obs = np.zeros(shape, np.float32)
+    with pytest.raises(clarity_gpu.ClarityError) as e:
+        net.predict(obs, 2.0, 4.0)
+    assert e.value.code == clarity_gpu.INVALID_ARGUMENT
+    # the handle still works at its previous shape
+    small, _ = _sample((8, 8, 16), 12)
+    assert np.isfinite(net.predict(small, 2.0, 4.0)).all()
