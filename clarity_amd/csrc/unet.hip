@@ -420,10 +420,33 @@ __device__ __forceinline__ void static_for(F &&f) {
 // branch, would make the wait for a unit's loads also wait for the loads
 // issued after it: vmcnt counts loads and stores in order).  Per output
 // voxel the sum runs over input planes z-1, z, z+1, then chunks, then
-// pairs.
+// pairs; where an upsampled chunk's two fine planes share a unit (below),
+// their pairs interleave.
 constexpr int kIB = 4;        // output planes per block
 constexpr int kIPDepth = 2;   // units whose loads are in flight
 constexpr bool kIPWLds = false;  // weights in LDS also for one chunk
+
+// k_conv_ip's units: (input slot s, chunk c), slot-major; slot s is input
+// plane z0 - 1 + s.  For an upsampled chunk the fine planes z0 + 2k and
+// z0 + 2k + 1 (slots 2k + 1 and 2k + 2; z0 is a multiple of the even kIB)
+// are one coarse plane, so one unit stages it once and feeds both slots'
+// output planes (two); plain chunks keep one unit per slot.
+struct IPUnit {
+  int s, c;
+  bool two;
+};
+template <int NCH, int NUP>
+constexpr IPUnit ip_unit(int u) {
+  int k = 0;
+  for (int s = 0; s < kIB + 2; s++)
+    for (int c = 0; c < NCH; c++) {
+      const bool up = c < NUP;
+      if (up && s % 2 == 0 && s >= 2 && s <= kIB) continue;  // merged into slot s - 1
+      if (k == u) return IPUnit{s, c, up && s % 2 == 1 && s < kIB};
+      k++;
+    }
+  return IPUnit{0, 0, false};
+}
 
 // POOL: also the 2x2x2 average pooling of the stored bfloat16 output into P
 // at half resolution (each channel sums its eight inputs in the order
@@ -435,7 +458,9 @@ __global__ void __launch_bounds__(256)
               const bf16 *__restrict__ Wf, const float *__restrict__ bias,
               bf16 *__restrict__ Y, Vol v, HeadArgs h, bf16 *__restrict__ P = nullptr) {
   static_assert(CIN % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)), "channel chunks of 16");
-  constexpr int COUT = 16, NCH = CIN / 16, NU = (kIB + 2) * NCH;
+  static_assert(kIB % 2 == 0, "coarse planes pair up");
+  constexpr int COUT = 16, NCH = CIN / 16, NUP = UP ? CA / 16 : 0;
+  constexpr int NU = (kIB + 2) * NCH - NUP * (kIB / 2);
   constexpr bool WLDS = NCH > 1 || kIPWLds;
   __shared__ uint4 buf[2 * 2 * kPS];
   __shared__ uint4 wl[WLDS ? NCH * 15 * 64 : 1];
@@ -482,21 +507,31 @@ __global__ void __launch_bounds__(256)
   const int y = y0 + wave;
   const Epi16 ep = epi16_load<HEAD>(bias, h, kq);
   f32x4 acc[kIB][4];
-  uint4 r[kIPDepth][kPL];
+  if constexpr (NUP > 0) {  // a merged unit starts two output planes
 #pragma unroll
-  for (int u = 0; u < kIPDepth && u < NU; u++)
-    zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 + u / NCH - 1, 16 * (u % NCH));
+    for (int o = 0; o < kIB; o++)
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) acc[o][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  uint4 r[kIPDepth][kPL];
+  static_for<kIPDepth>([&](auto uc) __attribute__((always_inline)) {
+    constexpr int u = decltype(uc)::value;
+    constexpr IPUnit U = ip_unit<NCH, NUP>(u);
+    if constexpr (u < NU) zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 + U.s - 1, 16 * U.c);
+  });
   // units one by one at compile time (a loop of 18 is not always unrolled,
   // and a unit index known only at run time would index register arrays)
   static_for<NU>([&](auto uc) __attribute__((always_inline)) {
     constexpr int u = decltype(uc)::value;
-    constexpr int s = u / NCH, c = u % NCH;  // input plane z0 - 1 + s, chunk c
+    constexpr IPUnit U = ip_unit<NCH, NUP>(u);
+    constexpr int s = U.s, c = U.c, ns = U.two ? 2 : 1;  // slots s .. s + ns - 1, chunk c
     zm_store(r[u % kIPDepth], buf, u & 1);
     __syncthreads();  // the unit is in LDS; (WLDS) so are the weights
-    if constexpr (u + kIPDepth < NU)
-      zm_load<CIN, CA, UP>(r[u % kIPDepth], t, X, S, v, z0 + (u + kIPDepth) / NCH - 1,
-                           16 * ((u + kIPDepth) % NCH));
-    if constexpr (c == 0 && s < kIB) {
+    if constexpr (u + kIPDepth < NU) {
+      constexpr IPUnit Un = ip_unit<NCH, NUP>(u + kIPDepth);
+      zm_load<CIN, CA, UP>(r[u % kIPDepth], t, X, S, v, z0 + Un.s - 1, 16 * Un.c);
+    }
+    if constexpr (NUP == 0 && c == 0 && s < kIB) {  // output plane s is first read here
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) acc[s][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
@@ -511,8 +546,9 @@ __global__ void __launch_bounds__(256)
     _Pragma("unroll") for (int mt = 0; mt < 4; mt++) fa[B][mt] =                          \
         __builtin_bit_cast(bf16x8, bb[foff[P] + mt * 16]);                                \
     _Pragma("unroll") for (int dz = -1; dz <= 1; dz++) {                                  \
-      const int o = s - 1 - dz; /* output plane z0 + o reads this plane at dz */          \
-      if (o < 0 || o >= kIB) continue;                                                    \
+      /* output plane z0 + o reads slot s + k at dz */                                    \
+      const int olo = s - 1 - dz, ohi = olo + ns - 1;                                     \
+      if (ohi < 0 || olo >= kIB) continue;                                                \
       if constexpr (WLDS)                                                                 \
         wa[B][dz + 1] = __builtin_bit_cast(bf16x8, wl[((c * 3 + dz + 1) * 5 + (P)) * 64 + lane]); \
     }                                                                                     \
@@ -523,14 +559,17 @@ __global__ void __launch_bounds__(256)
       if (p + 1 < 5) CLARITY_IP_READ_PAIR(p + 1, (p + 1) & 1)
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int dz = -1; dz <= 1; dz++) {
-        const int o = s - 1 - dz;
-        if (o < 0 || o >= kIB) continue;
+      for (int dz = -1; dz <= 1; dz++)
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++)
-          acc[o][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              WLDS ? wa[p & 1][dz + 1] : w[(dz + 1) * 5 + p], fa[p & 1][mt], acc[o][mt], 0, 0, 0);
-      }
+        for (int k = 0; k < ns; k++) {
+          const int o = s + k - 1 - dz;
+          if (o < 0 || o >= kIB) continue;
+#pragma unroll
+          for (int mt = 0; mt < 4; mt++)
+            acc[o][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                WLDS ? wa[p & 1][dz + 1] : w[(dz + 1) * 5 + p], fa[p & 1][mt], acc[o][mt], 0, 0,
+                0);
+        }
       __builtin_amdgcn_sched_barrier(0);
     }
 #undef CLARITY_IP_READ_PAIR
