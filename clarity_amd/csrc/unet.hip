@@ -631,12 +631,29 @@ __global__ void __launch_bounds__(256)
 
 constexpr int kCoDepth = 1;  // units whose loads are in flight (k_conv_co)
 
+// k_conv_co's units: (chunk c, input slot s), chunk-major; the slots of an
+// upsampled chunk pair up as in k_conv_ip (slots 2k + 1 and 2k + 2 are one
+// coarse plane: z0 is a multiple of the even IB).
+template <int NCH, int NUP, int IB>
+constexpr IPUnit co_unit(int u) {
+  int k = 0;
+  for (int c = 0; c < NCH; c++)
+    for (int s = 0; s < IB + 2; s++) {
+      const bool up = c < NUP;
+      if (up && s % 2 == 0 && s >= 2 && s <= IB) continue;  // merged into slot s - 1
+      if (k == u) return IPUnit{s, c, up && s % 2 == 1 && s < IB};
+      k++;
+    }
+  return IPUnit{0, 0, false};
+}
+
 // The same plane-by-plane scheme for more output channels (COUT/16 tiles,
 // each fragment feeding up to 3 x COUT/16 MFMAs), chunk-outer: the chunk's
 // weights (COUT/16 x 15 KB) sit in LDS while its IB + 2 input planes pass,
 // and every output plane of the block keeps its accumulators to the end.
 // Per output voxel the sum runs over chunks, then input planes z-1, z, z+1,
-// then pairs.
+// then pairs; an upsampled chunk's two fine planes that share a unit
+// (co_unit) interleave by pair.
 // POOL (IB = 2): also the 2x2x2 average pooling of the output into P, as
 // k_conv_ip's (the level's sizes are even, so a pooled voxel's rows and
 // planes are all inside the volume or all outside).
@@ -647,7 +664,9 @@ __global__ void __launch_bounds__(256)
               bf16 *__restrict__ Y, Vol v, bf16 *__restrict__ P = nullptr) {
   static_assert(CIN % 16 == 0 && COUT % 16 == 0 && (!UP || (CA % 16 == 0 && CA < CIN)),
                 "channel chunks of 16");
-  constexpr int NCT = COUT / 16, NCH = CIN / 16, NS = IB + 2, NU = NCH * NS;
+  static_assert(!UP || IB % 2 == 0, "coarse planes pair up");
+  constexpr int NCT = COUT / 16, NCH = CIN / 16, NS = IB + 2;
+  constexpr int NUP = UP ? CA / 16 : 0, NU = NCH * NS - NUP * (IB / 2);
   constexpr int NW = NCT * 15 * 64;  // weight fragments of a chunk
   __shared__ uint4 buf[2 * 2 * kPS];
   __shared__ uint4 wl[NW];  // [dz][pair][ct][lane]
@@ -675,12 +694,16 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) acc[o][ct][mt] = f32x4{0.f, 0.f, 0.f, 0.f};
   uint4 r[kCoDepth][kPL];
-#pragma unroll
-  for (int u = 0; u < kCoDepth && u < NU; u++)
-    zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 - 1 + u % NS, 16 * (u / NS));
+  static_for<kCoDepth>([&](auto uc) __attribute__((always_inline)) {
+    constexpr int u = decltype(uc)::value;
+    constexpr IPUnit U = co_unit<NCH, NUP, IB>(u);
+    if constexpr (u < NU) zm_load<CIN, CA, UP>(r[u], t, X, S, v, z0 - 1 + U.s, 16 * U.c);
+  });
   static_for<NU>([&](auto uc) __attribute__((always_inline)) {
     constexpr int u = decltype(uc)::value;
-    constexpr int c = u / NS, s = u % NS;  // chunk c, input plane z0 - 1 + s
+    constexpr IPUnit U = co_unit<NCH, NUP, IB>(u);
+    // chunk c, input slots s .. s + ns - 1 (input plane z0 - 1 + s)
+    constexpr int c = U.c, s = U.s, ns = U.two ? 2 : 1;
     zm_store(r[u % kCoDepth], buf, u & 1);
     __syncthreads();  // the unit is in LDS; the previous chunk's MFMAs are done
     if constexpr (s == 0) {
@@ -697,9 +720,10 @@ __global__ void __launch_bounds__(256)
       }
       __syncthreads();
     }
-    if constexpr (u + kCoDepth < NU)
-      zm_load<CIN, CA, UP>(r[u % kCoDepth], t, X, S, v, z0 - 1 + (u + kCoDepth) % NS,
-                           16 * ((u + kCoDepth) / NS));
+    if constexpr (u + kCoDepth < NU) {
+      constexpr IPUnit Un = co_unit<NCH, NUP, IB>(u + kCoDepth);
+      zm_load<CIN, CA, UP>(r[u % kCoDepth], t, X, S, v, z0 - 1 + Un.s, 16 * Un.c);
+    }
     const uint4 *bq = buf + (u & 1) * 2 * kPS;
     bf16x8 fa[2][4], wa[2][3][NCT];
 #define CLARITY_CO_READ_PAIR(P, B)                                                        \
@@ -707,8 +731,8 @@ __global__ void __launch_bounds__(256)
     _Pragma("unroll") for (int mt = 0; mt < 4; mt++) fa[B][mt] =                          \
         __builtin_bit_cast(bf16x8, bq[foff[P] + mt * 16]);                                \
     _Pragma("unroll") for (int dz = -1; dz <= 1; dz++) {                                  \
-      const int o = s - 1 - dz;                                                           \
-      if (o < 0 || o >= IB) continue;                                                     \
+      const int olo = s - 1 - dz, ohi = olo + ns - 1;                                     \
+      if (ohi < 0 || olo >= IB) continue;                                                 \
       _Pragma("unroll") for (int ct = 0; ct < NCT; ct++) wa[B][dz + 1][ct] =              \
           __builtin_bit_cast(bf16x8, wl[(((dz + 1) * 5 + (P)) * NCT + ct) * 64 + lane]);  \
     }                                                                                     \
@@ -719,16 +743,18 @@ __global__ void __launch_bounds__(256)
       if (p + 1 < 5) CLARITY_CO_READ_PAIR(p + 1, (p + 1) & 1)
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int dz = -1; dz <= 1; dz++) {
-        const int o = s - 1 - dz;  // output plane z0 + o reads this plane at dz
-        if (o < 0 || o >= IB) continue;
+      for (int dz = -1; dz <= 1; dz++)
 #pragma unroll
-        for (int ct = 0; ct < NCT; ct++)
+        for (int k = 0; k < ns; k++) {
+          const int o = s + k - 1 - dz;  // output plane z0 + o reads slot s + k at dz
+          if (o < 0 || o >= IB) continue;
 #pragma unroll
-          for (int mt = 0; mt < 4; mt++)
-            acc[o][ct][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                wa[p & 1][dz + 1][ct], fa[p & 1][mt], acc[o][ct][mt], 0, 0, 0);
-      }
+          for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+              acc[o][ct][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  wa[p & 1][dz + 1][ct], fa[p & 1][mt], acc[o][ct][mt], 0, 0, 0);
+        }
       __builtin_amdgcn_sched_barrier(0);
     }
 #undef CLARITY_CO_READ_PAIR
